@@ -1,0 +1,228 @@
+/*
+ * aco_hip.h - C ABI of the gfx950 ACO-CVRP kernels.
+ *
+ * This header replaces the CUDA-only kernel boundary of the reference,
+ * include/aco_cuda_kernels.h:82-181 (seven __global__ kernels and their
+ * `launch_*` host launchers, C++ linkage, caller-owned cudaMalloc buffers,
+ * void returns).  Here:
+ *   - every entry point is extern "C" and includable from C11;
+ *   - no HIP or CUDA type appears in a signature (opaque context, plain
+ *     pointers and sizes);
+ *   - the context owns every device buffer; the caller owns host buffers;
+ *   - every call returns an int status whose values 0..4 mirror AcoStatus
+ *     (include/aco.h:82-88); aco_hip_last_error() gives the message.
+ *
+ * A context holds a batch of instances (one instance is a batch of one).
+ * Each instance runs `colonies` independent colonies of `ants` ants.  Colony
+ * c of instance i has index i*colonies + c in the per-colony calls.
+ *
+ * Data conventions (identical on host, device and in the CPU oracle):
+ *   - nodes 0..n, node 0 is the depot; integer coordinates in [0, 32767];
+ *   - distances are TSPLIB EUC_2D nearest-integer: nint(sqrt(dx^2+dy^2));
+ *   - pheromone is the reference's log-quantized uint8 format
+ *     (src/cuda/aco_cuda_kernels.cu:25-45, src/cuda/aco_cuda.cu:209-242):
+ *     an (n+1)x(n+1) byte matrix with row stride n+1;
+ *   - a tour is the reference's step-interleaved route sequence
+ *     (src/cuda/aco_cuda.cu:138-168): 0, c.., 0, c.., 0, ... with K+1 depot
+ *     entries (consecutive routes share their boundary depot) and
+ *     length = visited customers + K + 1.
+ */
+#ifndef ACO_HIP_H
+#define ACO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; 0..4 are AcoStatus (include/aco.h:82-88). */
+enum {
+  ACO_HIP_OK = 0,
+  ACO_HIP_ERR_INVALID_INPUT = 1,
+  ACO_HIP_ERR_ALLOCATION = 2,
+  ACO_HIP_ERR_NO_SOLUTION = 3,
+  ACO_HIP_ERR_BACKEND = 4
+};
+
+/* Candidate-list ordering keys. */
+enum {
+  /* exact integer key (dx^2+dy^2, index): ties go to the lower index */
+  ACO_HIP_KNN_EXACT = 0,
+  /* the reference OpenMP key (float)sqrt((double)(dx^2+dy^2)) with strict
+   * '<' insertion, ties to the lower index (src/openmp-mpi/aco_parallel.c:218-250) */
+  ACO_HIP_KNN_REF_F32 = 1
+};
+
+/* Heuristic term used for candidate edges (north_star: learned prior). */
+enum {
+  ACO_HIP_ETA_DISTANCE = 0, /* eta^beta = (1/d)^beta, reference aco_cuda_kernels.cu:207-212 */
+  ACO_HIP_ETA_LEARNED = 1   /* per-candidate code set by aco_hip_set_candidate_eta() */
+};
+
+typedef struct AcoHipContext AcoHipContext;
+
+/* One CVRP instance (host memory, read during aco_hip_load only). */
+typedef struct {
+  int n;                 /* customers; nodes are 0..n */
+  int K;                 /* vehicles (routes per tour) */
+  int capacity;          /* Q, 1..32767 */
+  const int32_t *x;      /* n+1 coordinates, 0..32767 */
+  const int32_t *y;      /* n+1 coordinates, 0..32767 */
+  const int32_t *demand; /* n+1 demands, demand[0] == 0, customers 0..32767 */
+  uint32_t seed;         /* solver seed of this instance (Philox key) */
+} AcoHipInstance;
+
+/* Solver parameters shared by every instance of a context.
+ * Reference values: alpha 1, beta 2, rho 0.5, tau0 1, Q 1 (src/cuda/main_vrp.cu:53-57),
+ * depot close weight 2 (src/cuda/aco_cuda.cu:240), cand_k 32 (aco_cuda.cu:198). */
+typedef struct {
+  int cand_k;            /* candidate list length k, 1..64 (clamped to n) */
+  int ants;              /* ants per colony m, >= 1 */
+  int colonies;          /* colonies per instance, >= 1 */
+  float alpha;           /* pheromone exponent (any real value) */
+  float beta;            /* heuristic exponent (any real value) */
+  float rho;             /* evaporation factor in (0, 1) */
+  float tau0;            /* initial pheromone */
+  float Q;               /* deposit factor; deposits use 100*Q (aco_cuda.cu:228) */
+  float depot_close_weight;
+  int knn_key;           /* ACO_HIP_KNN_* */
+  int ants_per_group;    /* ants (wavefronts) per workgroup, 0 = auto */
+  int stage_lds;         /* -1 auto, 0 read candidate rows from global memory, 1 stage in LDS */
+} AcoHipParams;
+
+/* Device-side state of one colony, read back with aco_hip_get_colony(). */
+typedef struct {
+  int32_t epoch;            /* epochs executed */
+  int32_t iter_since_best;  /* reference iter_since_best (aco_cuda.cu:322-399) */
+  int32_t halted;           /* stop rule reached; later epochs are no-ops */
+  int32_t found;            /* last epoch had a feasible ant */
+  int32_t best_ant;         /* last epoch's iteration-best ant or -1 */
+  int32_t improved;         /* last epoch improved the global best */
+  int32_t best_len;         /* length of the global-best tour */
+  int32_t feasible_ants;    /* feasible ants in the last epoch */
+  int64_t iter_best_cost;   /* last epoch's best feasible cost or INT64_MAX */
+  int64_t global_best_cost; /* best feasible cost so far or INT64_MAX */
+  float dep_iter;           /* last iteration-best deposit 0.3*100Q/L_it */
+  float dep_best;           /* last global-best deposit 0.7*100Q/L* */
+} AcoHipColonyState;
+
+/* Stop rules of the epoch loop (src/common/aco_config.c:33-73). */
+typedef struct {
+  double timeout_seconds;     /* 0 = off; checked between epoch chunks */
+  int stagnation_epochs;      /* 0 = off */
+  double min_rel_improvement; /* fraction, reference default 1e-3 */
+  int fixed_epochs;           /* >0: run exactly this many epochs */
+  int max_epochs;             /* >0: hard cap (safety), 0 = none */
+  double progress_interval_seconds; /* stderr heartbeat, 0 = off */
+  int print_progress;         /* print the reference's [progress] lines */
+  int chunk_epochs;           /* epochs per launch batch / graph replay, 0 = auto */
+  int use_graph;              /* capture a chunk of epochs in a hipGraph */
+} AcoHipRunConfig;
+
+typedef struct {
+  int epochs;               /* epochs executed by the first colony */
+  double seconds;           /* wall seconds of the epoch loop */
+  double ant_tours_per_s;   /* sum over colonies of ants*epochs / seconds */
+} AcoHipRunStats;
+
+/* Per-kernel timing from hipEvents recorded on the context stream. */
+typedef struct {
+  double construct_ms;  int construct_launches;
+  double best_ms;       int best_launches;
+  double evaporate_ms;  int evaporate_launches;
+  double deposit_ms;    int deposit_launches;
+  double knn_ms;        int knn_launches;
+} AcoHipTiming;
+
+/* ---- context ------------------------------------------------------------ */
+int aco_hip_create(AcoHipContext **out, int device);
+void aco_hip_destroy(AcoHipContext *ctx);
+const char *aco_hip_last_error(const AcoHipContext *ctx);
+int aco_hip_sync(AcoHipContext *ctx);
+
+/* Uploads instances, builds the quantization and heuristic tables and
+ * allocates every pool.  Replaces the cudaMalloc/cudaMemcpy block of
+ * src/cuda/aco_cuda.cu:209-286. */
+int aco_hip_load(AcoHipContext *ctx, const AcoHipInstance *inst, int num_instances,
+                 const AcoHipParams *params);
+int aco_hip_num_colonies(const AcoHipContext *ctx);
+/* Effective per-context sizes after clamping: k, ants, tour stride, ants/group, lds flag. */
+int aco_hip_get_shape(const AcoHipContext *ctx, int *cand_k, int *ants, int *tour_stride,
+                      int *ants_per_group, int *stage_lds);
+
+/* ---- kernels (one launch each, asynchronous on the context stream) ------ */
+/* kNN candidate lists: replaces launch_build_candidate_lists
+ * (include/aco_cuda_kernels.h:156, kernel aco_cuda_kernels.cu:164-214). */
+int aco_hip_build_candidates(AcoHipContext *ctx);
+/* Fill every pheromone byte with q(tau0) and reset the colony state:
+ * replaces launch_init_tau (include/aco_cuda_kernels.h:151). */
+int aco_hip_init_tau(AcoHipContext *ctx);
+/* Reset ants and build one tour per ant for the current device epoch:
+ * replaces launch_reset_ant_state + launch_construct_solutions
+ * (include/aco_cuda_kernels.h:161,166; kernels aco_cuda_kernels.cu:227-562). */
+int aco_hip_construct(AcoHipContext *ctx);
+/* Iteration best, global best, stagnation and stop bookkeeping on device:
+ * replaces select_iter_best_host and the host block of aco_cuda.cu:351-399. */
+int aco_hip_select_best(AcoHipContext *ctx);
+/* Saturating quantized evaporation: replaces launch_evaporate_tau
+ * (include/aco_cuda_kernels.h:171, kernel aco_cuda_kernels.cu:145-155). */
+int aco_hip_evaporate(AcoHipContext *ctx);
+/* Iteration-best deposit: replaces launch_deposit_solution (aco_cuda_kernels.h:176). */
+int aco_hip_deposit_iter(AcoHipContext *ctx);
+/* Global-best deposit: replaces launch_deposit_flat_solution (aco_cuda_kernels.h:181). */
+int aco_hip_deposit_best(AcoHipContext *ctx);
+/* `count` full epochs: construct, select_best, evaporate, deposit_iter, deposit_best. */
+int aco_hip_epochs(AcoHipContext *ctx, int count);
+
+/* ---- epoch loop: replaces aco_vrp_cuda_with_capacity (aco_cuda.cu:188-454) - */
+int aco_hip_run(AcoHipContext *ctx, const AcoHipRunConfig *cfg, AcoHipRunStats *stats);
+
+/* ---- downloads / uploads (synchronous) ---------------------------------- */
+/* (n+1)*k candidate indices, nint distances and eta^beta values (any may be NULL). */
+int aco_hip_get_candidates(AcoHipContext *ctx, int instance, int32_t *idx, int32_t *dist,
+                           float *eta);
+/* Replace the per-candidate heuristic (learned prior); eta has (n+1)*k values > 0. */
+int aco_hip_set_candidate_eta(AcoHipContext *ctx, int instance, const float *eta);
+/* (n+1)^2 pheromone bytes and (n+1)*k candidate-aligned copy (either may be NULL). */
+int aco_hip_get_tau(AcoHipContext *ctx, int colony, uint8_t *tau, uint8_t *tau_cand);
+int aco_hip_set_tau(AcoHipContext *ctx, int colony, const uint8_t *tau);
+/* tours: ants*tour_stride int32; len, unvisited: ants int32; cost: ants int64. */
+int aco_hip_get_tours(AcoHipContext *ctx, int colony, int32_t *tours, int32_t *len,
+                      int64_t *cost, int32_t *unvisited);
+int aco_hip_get_colony(AcoHipContext *ctx, int colony, AcoHipColonyState *st);
+int aco_hip_set_epoch(AcoHipContext *ctx, int colony, int epoch);
+int aco_hip_get_best(AcoHipContext *ctx, int colony, int32_t *tour, int32_t *len,
+                     int64_t *cost);
+/* Deposit a host tour on one colony (sequential edge semantics of
+ * kernel_deposit_flat_solution, aco_cuda_kernels.cu:637-668). */
+int aco_hip_deposit_tour(AcoHipContext *ctx, int colony, const int32_t *tour, int len,
+                         float delta);
+/* Route-cost reduction of host tours of instance `instance` (int64 nint sums);
+ * the reference accumulates it in-kernel (aco_cuda_kernels.cu:540,550). */
+int aco_hip_route_costs(AcoHipContext *ctx, int instance, const int32_t *tours,
+                        const int32_t *len, int count, int stride, int64_t *cost);
+/* Host copies of the shared tables: deq[256], thr[255], taupow[256], eta[eta_len]. */
+int aco_hip_get_tables(AcoHipContext *ctx, float *deq, float *thr, float *taupow,
+                       float *eta_lut, int *eta_len, uint8_t *q_tau0, uint8_t *q_evap);
+
+/* ---- timing ------------------------------------------------------------- */
+int aco_hip_set_timing(AcoHipContext *ctx, int enable);
+int aco_hip_get_timing(AcoHipContext *ctx, AcoHipTiming *t);
+
+/* ---- one-call solver (drop-in for aco_vrp_cuda_with_capacity,
+ *      src/cuda/main_vrp.cu:29-35): one instance, one colony.  Writes the
+ *      best tour (capacity n+K+1), its length and its cost.  Stop rules
+ *      come from the ACO_SOLVER_* environment like the reference. ------- */
+int aco_vrp_hip_with_capacity(int n, int K, int capacity, int m, const int32_t *x,
+                              const int32_t *y, const int32_t *demand, double alpha,
+                              double beta, double rho, double tau0, double Q,
+                              unsigned int seed, int32_t *best_tour, int32_t *best_len,
+                              int64_t *best_cost);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACO_HIP_H */

@@ -1,0 +1,89 @@
+/*
+ * aco_inline.h - inline arithmetic shared by the host library and the gfx950
+ * kernels, so that both evaluate bit-identical values:
+ *
+ *  - Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+ *    "Parallel random numbers: as easy as 1, 2, 3", SC'11), written from the
+ *    published round function.  It replaces the per-ant LCG of
+ *    src/cuda/aco_cuda_kernels.cu:13-16,253.  Draw layout of the solver:
+ *      key = {instance seed, ACO_PHILOX_KEY1}
+ *      ctr = {decision index within the ant's tour, ant, epoch, colony}
+ *      out[0] -> move roulette, out[1] -> depot-close roulette;
+ *  - TSPLIB EUC_2D nearest-integer distance (exact integer square root);
+ *  - pheromone quantization through a threshold table.
+ */
+#ifndef ACO_INLINE_H
+#define ACO_INLINE_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define ACO_HD __host__ __device__
+#else
+#define ACO_HD
+#endif
+
+#define ACO_PHILOX_M0 0xD2511F53u
+#define ACO_PHILOX_M1 0xCD9E8D57u
+#define ACO_PHILOX_W0 0x9E3779B9u
+#define ACO_PHILOX_W1 0xBB67AE85u
+#define ACO_PHILOX_KEY1 0x5CA1AB1Eu
+
+static inline ACO_HD void aco_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                            uint32_t k0, uint32_t k1, uint32_t out[4]) {
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)ACO_PHILOX_M0 * c0;
+    uint64_t p1 = (uint64_t)ACO_PHILOX_M1 * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += ACO_PHILOX_W0;
+    k1 += ACO_PHILOX_W1;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+/* Uniform float in [0,1) from the top 24 bits (exact conversion). */
+static inline ACO_HD float aco_u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+
+/* Exact floor(sqrt(s)) for s < 2^32 - 2^17: float estimate, then integer fix-up. */
+static inline ACO_HD uint32_t aco_isqrt_u32(uint32_t s) {
+  uint32_t r = (uint32_t)__builtin_sqrtf((float)s);
+  if ((uint64_t)r * r > s) --r;
+  if ((uint64_t)(r + 1) * (r + 1) <= s) ++r;
+  return r;
+}
+
+/* nint(sqrt(s)): sqrt(s) >= r + 1/2  <=>  s > r^2 + r for integers. */
+static inline ACO_HD int32_t aco_nint_sqrt_u32(uint32_t s) {
+  uint32_t r = aco_isqrt_u32(s);
+  return (int32_t)(r + (s > r * r + r ? 1u : 0u));
+}
+
+/* TSPLIB EUC_2D distance for coordinates in [0, 32767]. */
+static inline ACO_HD int32_t aco_nint_dist(int32_t x1, int32_t y1, int32_t x2, int32_t y2) {
+  int32_t dx = x1 - x2, dy = y1 - y2;
+  uint32_t s = (uint32_t)(dx * dx) + (uint32_t)(dy * dy);
+  return aco_nint_sqrt_u32(s);
+}
+
+/* Quantize tau through the 255 increasing thresholds thr[b] (the smallest
+ * float whose level is >= b+1): level = number of thresholds <= tau. */
+static inline ACO_HD uint32_t aco_quantize_thr(const float *thr, float tau) {
+  if (!(tau > 0.0f)) return 0u;
+  uint32_t lo = 0u, hi = 255u;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (tau >= thr[mid]) lo = mid + 1u;
+    else hi = mid;
+  }
+  return lo;
+}
+
+#endif /* ACO_INLINE_H */

@@ -1,0 +1,100 @@
+/*
+ * aco_kernels.h - device data layout shared by the gfx950 kernels and the
+ * host side of the C ABI (internal; the public boundary is include/aco_hip.h).
+ *
+ * HBM layout of a context (all pools 256-byte aligned):
+ *   xy      int2   [sum(n_i+1) padded]   integer coordinates per node
+ *   dem     uint16 [same]                demand per node
+ *   dscore  float  [same]                depot-close score 2*(1/d(i,0))^beta
+ *   rec     uint32 [sum((n_i+1)*k)]      candidate records idx | nint(d) << 16
+ *   pher    uint8  per colony: (n+1)^2 pheromone matrix (row stride n+1,
+ *                  the reference format), then the (n+1)*k candidate-aligned
+ *                  copy tau_cand[i][t] == tau[i][cand[i][t]], 16-byte padded
+ *   tours   int32  [colonies][ants][stride] step-interleaved routes
+ *   len/unvis int32, cost int64 [colonies*ants]
+ *   best    int32  [colonies][stride]   global-best tour per colony
+ *   tables  deq[256], thr[255], taupow[256], eta[dmax+1]
+ */
+#ifndef ACO_KERNELS_H
+#define ACO_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "aco_hip.h"
+
+struct InstDesc {
+  int n, K, k, Q;
+  int node_off;        /* into xy/dem/dscore (multiple of 4) */
+  int row_base;        /* first kNN row of this instance */
+  long long total_demand;
+  long long cand_off;  /* into rec (multiple of 4) */
+  uint32_t seed;
+  int pad;
+};
+
+struct ColonyDesc {
+  int inst;
+  int colony_in_inst;  /* Philox ctr[3] */
+  int m;
+  int stride;          /* tour stride (multiple of 64) */
+  long long pher_off;  /* bytes into pher (multiple of 256) */
+  long long tcand_off; /* bytes into pher (multiple of 16) */
+  long long pher_bytes;/* whole block, multiple of 16 */
+  long long tour_off;  /* int32 elements into tours */
+  int ant_off;         /* into len/cost/unvis */
+  int best_off;        /* int32 elements into best */
+};
+
+struct DevCtx {
+  const InstDesc *inst;
+  const ColonyDesc *col;
+  AcoHipColonyState *state;
+  const int *row_inst;
+  const int2 *xy;
+  const uint16_t *dem;
+  float *dscore;
+  uint32_t *rec;
+  uint8_t *pher;
+  int32_t *tours;
+  int32_t *len;
+  long long *cost;
+  int32_t *unvis;
+  int32_t *best;
+  const float *deq;
+  const float *thr;
+  const float *taupow;
+  const float *eta;
+  int eta_len;
+  uint32_t q_evap;
+  float qdep;
+  int fixed_epochs;
+  int stagnation;
+  int max_epochs;
+  double min_rel;
+};
+
+static inline __host__ __device__ size_t round16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+/* is_significant_improvement (src/cuda/aco_cuda.cu:83-94), ACO_EPS = 1e-9. */
+static inline __host__ __device__ int aco_significant(double prev, double now, double min_rel) {
+  const double eps = 1e-9;
+  if (prev >= 1.7976931348623157e308 || now >= 1.7976931348623157e308) return now < prev;
+  if (now >= prev - eps) return 0;
+  const double rel = (prev - now) / (prev > eps ? prev : eps);
+  return rel + eps >= min_rel;
+}
+
+/* kernels (aco_kernels.hip) */
+__global__ void k_knn(DevCtx P, int total_rows, int knn_key, float depot_w);
+__global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
+template <bool STAGE> __global__ void k_construct(DevCtx P, int groups_per_colony, int W);
+__global__ void k_select_best(DevCtx P);
+__global__ void k_evaporate(DevCtx P);
+__global__ void k_deposit(DevCtx P, int which);
+__global__ void k_deposit_tour(DevCtx P, int colony, const int32_t *tour, int len, float delta);
+__global__ void k_route_costs(DevCtx P, int instance, const int32_t *tours, const int32_t *lens,
+                              int count, int stride, long long *out);
+
+#endif
