@@ -140,6 +140,8 @@ void aco_tau_tables_build(AcoTauTables *t, float alpha, float rho, float tau0);
 void aco_eta_table_build(float *eta, int dmax, float beta);
 /* Quantize through the threshold table (number of thresholds <= tau). */
 uint8_t aco_tau_quantize(const AcoTauTables *t, float tau);
+/* Host entry to the shared Philox4x32-10 (include/aco_inline.h). */
+void aco_philox_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,33 @@
+# Builds the reference's CPU paths from their sources under /root/reference
+# (read in place, never copied) into oracle/_ref/.  Used only to pin the
+# oracle (golden vectors) and as a timed CPU baseline in this container.
+#   make -C oracle -f ref.mk
+# Same flags as the reference's own build (tools/makefile/build.mk): gcc -O3
+# -std=c11, OpenMP for the parallel path, no MPI (mpicc is absent here).
+
+REF    ?= /root/reference
+OUT    ?= _ref
+CC     ?= gcc
+CFLAGS := -O3 -std=c11 -D_GNU_SOURCE -I$(REF)/include
+COMMON := $(REF)/src/common/aco_config.c $(REF)/src/common/aco_shared.c \
+          $(REF)/src/common/cli_common.c $(REF)/src/common/instance_parser.c \
+          $(REF)/src/common/matrix.c $(REF)/src/common/solution.c
+
+all: $(OUT)/aco_seq $(OUT)/aco_omp $(OUT)/ref_knn
+
+$(OUT):
+	mkdir -p $@
+
+$(OUT)/aco_seq: | $(OUT)
+	$(CC) $(CFLAGS) -o $@ $(REF)/src/main.c $(REF)/src/seq/aco_sequential.c $(COMMON) -lm
+
+$(OUT)/aco_omp: | $(OUT)
+	$(CC) $(CFLAGS) -fopenmp -o $@ $(REF)/src/main.c $(REF)/src/openmp-mpi/aco_parallel.c $(COMMON) -lm
+
+$(OUT)/ref_knn: ref_knn_driver.c | $(OUT)
+	$(CC) $(CFLAGS) -fopenmp -I$(REF)/src/openmp-mpi -o $@ ref_knn_driver.c $(COMMON) -lm
+
+clean:
+	rm -rf $(OUT)
+
+.PHONY: all clean

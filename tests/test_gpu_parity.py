@@ -1,0 +1,222 @@
+"""GPU parity: every kernel of the hot path against the CPU oracle, called
+through the C ABI.  Integer, byte and index outputs are compared bit-exactly;
+the float tables too (they are shared bit patterns, not recomputed)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from vehicleroutingproblem_amd import HipSolver, Instance, Params
+from vehicleroutingproblem_amd import _native as N
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def smoke_instance():
+    """The 8-customer instance of the reference smoke test
+    (tools/bash/smoke_test.sh:17-48): unit demand, K=2, Q=5."""
+    x = [50, 10, 20, 10, 20, 80, 90, 80, 90]
+    y = [50, 10, 10, 20, 20, 80, 80, 90, 90]
+    d = [0] + [1] * 8
+    return Instance(np.array(x), np.array(y), np.array(d), 2, 5, "smoke")
+
+
+CASES = {
+    "c1_n100": (lambda: Instance.uniform(100, 8, seed=1234), dict(cand_k=15, ants=64, rho=0.1)),
+    "n300_clustered": (lambda: Instance.clustered(300, 10, side=10000, sigma=300, dmin=1,
+                                                  dmax=100, slack_pct=105, seed=7),
+                       dict(cand_k=32, ants=96, rho=0.1)),
+    "smoke_unit": (smoke_instance, dict(cand_k=32, ants=32, rho=0.5)),
+    "tiny_n5": (lambda: Instance.uniform(5, 2, seed=3), dict(cand_k=20, ants=16, rho=0.5)),
+    "n1030_two_chunks": (lambda: Instance.uniform(1030, 9, seed=11),
+                         dict(cand_k=20, ants=64, rho=0.1)),
+}
+
+
+def make_pair(name, **over):
+    mk, kw = CASES[name]
+    inst = mk()
+    kw = dict(kw)
+    kw.update(over)
+    seed = kw.pop("seed", 1234)
+    p = Params(**kw)
+    s = HipSolver(inst, p, seeds=[seed])
+    o = O.Oracle(inst, cand_k=p.cand_k, ants=p.ants, alpha=p.alpha, beta=p.beta, rho=p.rho,
+                 tau0=p.tau0, Q=p.Q, depot_w=p.depot_close_weight, knn_key=p.knn_key, seed=seed)
+    return inst, s, o
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_tables_and_candidates(gpu, name):
+    inst, s, o = make_pair(name)
+    s.build_candidates()
+    tg, to = s.tables(), o.tables()
+    for key in ("deq", "thr", "taupow", "eta"):
+        assert np.array_equal(tg[key].view(np.uint32), to[key].view(np.uint32)), key
+    assert tg["q_tau0"] == to["q_tau0"] and tg["q_evap"] == to["q_evap"]
+    gi, gd, ge = s.candidates()
+    oi, od, oe = o.candidates()
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gd, od)
+    assert np.array_equal(ge.view(np.uint32), oe.view(np.uint32))
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["c1_n100", "n300_clustered"])
+def test_candidates_reference_key(gpu, name):
+    inst, s, o = make_pair(name, knn_key=N.KNN_REF_F32)
+    s.build_candidates()
+    assert np.array_equal(s.candidates()[0], o.candidates()[0])
+    s.close()
+
+
+def _compare_tours(s, o):
+    gt, gl, gc, gu = s.tours()
+    ot, ol, oc, ou = o.tours()
+    assert np.array_equal(gl, ol)
+    for a in range(len(gl)):
+        assert np.array_equal(gt[a, :gl[a]], ot[a, :ol[a]]), f"ant {a}"
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(gu, ou)
+    return gt, gl, gc, gu
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("stage", [1, 0])
+def test_construct_matches_oracle(gpu, name, stage):
+    inst, s, o = make_pair(name, stage_lds=stage)
+    s.prepare()
+    s.construct()
+    o.construct()
+    gt, gl, gc, gu = _compare_tours(s, o)
+    # every tour is a feasible-or-reported route set with exact integer cost
+    for a in range(len(gl)):
+        t = gt[a, :gl[a]]
+        assert t[0] == 0 and t[-1] == 0 and (t == 0).sum() == inst.K + 1
+        cust = t[t > 0]
+        assert len(np.unique(cust)) == len(cust) == inst.n - gu[a]
+        assert o.route_cost(t) == gc[a]
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["c1_n100", "n300_clustered", "smoke_unit", "n1030_two_chunks"])
+def test_epochs_match_oracle(gpu, name):
+    inst, s, o = make_pair(name)
+    s.prepare()
+    for e in range(6):
+        s.epochs(1)
+        o.epochs(1)
+        gs, os_ = s.colony_state(), o.state()
+        for key in ("epoch", "iter_since_best", "found", "best_ant", "best_len",
+                    "feasible_ants", "iter_best_cost", "global_best_cost", "improved"):
+            assert gs[key] == os_[key], (e, key, gs[key], os_[key])
+        assert np.float32(gs["dep_iter"]) == np.float32(os_["dep_iter"])
+        tg, cg = s.tau()
+        to, co = o.tau()
+        assert np.array_equal(tg, to), f"tau differs after epoch {e}"
+        assert np.array_equal(cg, co), f"tau_cand differs after epoch {e}"
+    if o.best()[1] is not None:
+        bt, bc = s.best()
+        ot, oc = o.best()
+        assert bc == oc and np.array_equal(bt, ot)
+    s.close()
+
+
+def test_ants_per_group_invariance(gpu):
+    ref = None
+    for apg in (1, 3, 8):
+        inst, s, o = make_pair("c1_n100", ants_per_group=apg)
+        s.prepare()
+        s.epochs(3)
+        t = s.tours()
+        if ref is None:
+            ref = t
+        else:
+            for a, b in zip(ref, t):
+                assert np.array_equal(a, b)
+        s.close()
+
+
+def test_route_costs(gpu):
+    inst, s, o = make_pair("n300_clustered")
+    s.prepare()
+    s.construct()
+    tours, lens, cost, _ = s.tours()
+    got = s.route_costs(tours, lens)
+    assert np.array_equal(got, cost)
+    rng = np.random.default_rng(5)
+    perm = np.concatenate([[0], rng.permutation(np.arange(1, inst.n + 1)), [0]]).astype(np.int32)
+    got = s.route_costs(perm[None, :], np.array([len(perm)], np.int32))
+    assert got[0] == o.route_cost(perm)
+    s.close()
+
+
+def test_deposit_tour_conflicts(gpu):
+    """Single-customer routes (0,c,0) and empty routes (0,0) touch the same
+    pheromone entries twice: the result must equal the sequential order."""
+    inst, s, o = make_pair("c1_n100")
+    s.prepare()
+    rng = np.random.default_rng(9)
+    tau = rng.integers(0, 256, size=(inst.n + 1, inst.n + 1), dtype=np.uint8)
+    s.set_tau(tau)
+    o.set_tau(tau)
+    tour = [0, 5, 0, 0, 7, 9, 11, 0, 0, 13, 0, 2, 3, 0]
+    for delta in (0.0013, 0.05, 3.0):
+        s.deposit_tour(tour, delta)
+        o.deposit_tour(tour, delta)
+        assert np.array_equal(s.tau()[0], o.tau()[0])
+        assert np.array_equal(s.tau()[1], o.tau()[1])
+    s.close()
+
+
+def test_run_stagnation_matches_oracle(gpu):
+    inst, s, o = make_pair("c1_n100")
+    s.prepare()
+    r = s.run(stagnation_epochs=15, min_rel_improvement=1e-3, max_epochs=400, chunk_epochs=8,
+              use_graph=True)
+    o.set_stop_rules(stagnation=15, min_rel=1e-3)
+    o.epochs(400)
+    gs, os_ = s.colony_state(), o.state()
+    assert gs["epoch"] == os_["epoch"] == r["epochs"]
+    assert gs["global_best_cost"] == os_["global_best_cost"]
+    assert np.array_equal(s.best()[0], o.best()[0])
+    assert np.array_equal(s.tau()[0], o.tau()[0])
+    s.close()
+
+
+def test_batch_matches_single(gpu):
+    """Instances batched in one context with two colonies each give the
+    tours of each instance run alone (the Philox stream depends only on the
+    instance seed, the colony index, the epoch, the ant and the decision)."""
+    insts = [Instance.uniform(120, 6, seed=21), Instance.clustered(200, 8, seed=22)]
+    p = Params(cand_k=16, ants=32, colonies=2, rho=0.1)
+    b = HipSolver(insts, p, seeds=[101, 202])
+    b.prepare()
+    b.epochs(3)
+    for i, inst in enumerate(insts):
+        single = HipSolver(inst, Params(cand_k=16, ants=32, colonies=1, rho=0.1),
+                           seeds=[[101, 202][i]])
+        single.prepare()
+        single.epochs(3)
+        ts = single.tours(0)
+        tb = b.tours(i * 2)
+        for x, y in zip(ts, tb):
+            assert np.array_equal(x, y)
+        assert np.array_equal(single.tau(0)[0], b.tau(i * 2)[0])
+        single.close()
+    b.close()
+
+
+def test_cli_smoke_instance(gpu, tmp_path):
+    path = tmp_path / "smoke.vrp"
+    smoke_instance().write(str(path))
+    env = dict(os.environ, ACO_SOLVER_TIMEOUT_SECONDS="10", ACO_SOLVER_STAGNATION_EPOCHS="10")
+    out = subprocess.run([N.CLI_PATH, str(path), "2", "0", "1234"], env=env,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    assert sum(1 for l in lines if l.startswith("Route ")) == 2
+    assert any(l.startswith("Cost: ") for l in lines)
+    assert any(l.startswith("best cost: ") for l in lines)

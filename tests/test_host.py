@@ -1,0 +1,150 @@
+"""CPU tests of the host surface: .vrp reader/writer, generators, runtime
+configuration, CLI argument handling and error exits, and the C ABI library
+exporting every symbol its headers declare.  No GPU needed."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from vehicleroutingproblem_amd import Instance, _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", text)) -
+                  {"if", "for", "while", "return", "sizeof", "defined"})
+
+
+@pytest.mark.parametrize("header", ["aco_hip.h", "aco_host.h"])
+def test_library_exports_declared_symbols(header):
+    lib = C.CDLL(N.LIB_PATH)
+    names = _declared(header)
+    assert len(names) > 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_table_covers_c_abi():
+    assert set(_declared("aco_hip.h")) <= set(N.HIP_SYMBOLS)
+
+
+def test_reader_reference_instances():
+    inst = Instance.read(os.path.join(GOLDEN, "ref_n500_k8_s19000.vrp"), require_unit_demand=True)
+    assert inst.n == 500 and inst.K == 8 and inst.capacity == 75
+    assert inst.x[0] == 38 and inst.y[0] == 49 and inst.x[500] == 21 and inst.y[500] == 7
+    assert inst.demand[0] == 0 and np.all(inst.demand[1:] == 1)
+    smoke = Instance.read(os.path.join(GOLDEN, "ref_smoke.vrp"))
+    assert smoke.n == 8 and smoke.K == 2 and smoke.capacity == 5
+
+
+def test_writer_round_trip(tmp_path):
+    inst = Instance.clustered(300, 10, seed=9)
+    p = tmp_path / "c.vrp"
+    inst.write(str(p))
+    back = Instance.read(str(p))
+    assert back.K == inst.K and back.capacity == inst.capacity
+    for a in ("x", "y", "demand"):
+        assert np.array_equal(getattr(back, a), getattr(inst, a))
+    with pytest.raises(Exception):
+        Instance.read(str(p), require_unit_demand=True)  # weighted demand
+
+
+BAD = {
+    "no_vehicles": "DIMENSION: 3\nCAPACITY: 5\nNODE_COORD_SECTION\n1 0 0\n2 1 1\n3 2 2\n"
+                   "DEMAND_SECTION\n1 0\n2 1\n3 1\nEOF\n",
+    "depot_demand": "DIMENSION: 3\nVEHICLES: 1\nCAPACITY: 5\nNODE_COORD_SECTION\n1 0 0\n2 1 1\n"
+                    "3 2 2\nDEMAND_SECTION\n1 2\n2 1\n3 1\nEOF\n",
+    "short_coords": "DIMENSION: 3\nVEHICLES: 1\nCAPACITY: 5\nNODE_COORD_SECTION\n1 0 0\n2 1 1\n",
+    "fraction_coord": "DIMENSION: 3\nVEHICLES: 1\nCAPACITY: 5\nNODE_COORD_SECTION\n1 0 0\n"
+                      "2 1.5 1\n3 2 2\nDEMAND_SECTION\n1 0\n2 1\n3 1\nEOF\n",
+    "dim_after": "VEHICLES: 1\nCAPACITY: 5\nNODE_COORD_SECTION\n1 0 0\nDIMENSION: 2\n",
+}
+
+
+@pytest.mark.parametrize("case", list(BAD))
+def test_reader_rejects_malformed(tmp_path, case):
+    p = tmp_path / f"{case}.vrp"
+    p.write_text(BAD[case])
+    with pytest.raises(Exception):
+        Instance.read(str(p))
+
+
+def test_generators_deterministic_and_capacity():
+    a = Instance.uniform(1000, 8, side=1000, dmin=1, dmax=10, slack_pct=110, seed=1)
+    b = Instance.uniform(1000, 8, side=1000, dmin=1, dmax=10, slack_pct=110, seed=1)
+    c = Instance.uniform(1000, 8, side=1000, dmin=1, dmax=10, slack_pct=110, seed=2)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.demand, b.demand)
+    assert not np.array_equal(a.x, c.x)
+    assert a.x[0] == 500 and a.y[0] == 500
+    assert a.x.min() >= 0 and a.x.max() <= 1000 and a.demand[1:].min() >= 1
+    assert a.demand[1:].max() <= 10 and a.demand[0] == 0
+    total = a.total_demand
+    assert a.capacity == -(-11 * total // 80)  # ceil(1.1 * sum / 8)
+    # uniformity sanity: every coordinate decile is populated about equally
+    hist = np.histogram(a.x[1:], bins=10, range=(0, 1001))[0]
+    assert hist.min() > 60
+    cl = Instance.clustered(5000, 40, side=10000, clusters=25, sigma=300, dmin=1, dmax=100,
+                            slack_pct=105, seed=3)
+    assert cl.x[0] == 5000 and cl.x.max() <= 10000 and cl.demand[1:].max() <= 100
+    assert cl.capacity == -(-105 * cl.total_demand // 4000)
+    # clustered: most points are near one of few centres (low spread per cluster)
+    assert np.unique((cl.x[1:] // 500) * 100 + cl.y[1:] // 500).size < 300
+
+
+def test_runtime_config_env(monkeypatch):
+    lib = N.load()
+    for k in ("ACO_SOLVER_TIMEOUT_SECONDS", "ACO_SOLVER_STAGNATION_EPOCHS",
+              "ACO_SOLVER_MIN_REL_IMPROVEMENT", "ACO_SOLVER_FIXED_EPOCHS",
+              "ACO_SOLVER_PROGRESS_INTERVAL_SECONDS"):
+        monkeypatch.delenv(k, raising=False)
+    cfg = N.AcoRuntimeConfig()
+    lib.aco_runtime_config_load_env(C.byref(cfg))
+    assert cfg.timeout_seconds == 0 and cfg.stagnation_epochs == 0
+    assert cfg.min_rel_improvement == pytest.approx(1e-3) and cfg.progress_interval_seconds == 10
+    monkeypatch.setenv("ACO_SOLVER_TIMEOUT_SECONDS", "2.5")
+    monkeypatch.setenv("ACO_SOLVER_STAGNATION_EPOCHS", "80")
+    monkeypatch.setenv("ACO_SOLVER_MIN_REL_IMPROVEMENT", "0.1")  # percent
+    monkeypatch.setenv("ACO_SOLVER_FIXED_EPOCHS", "-3")
+    monkeypatch.setenv("ACO_SOLVER_PROGRESS_INTERVAL_SECONDS", "-1")
+    lib.aco_runtime_config_load_env(C.byref(cfg))
+    assert cfg.timeout_seconds == 2.5 and cfg.stagnation_epochs == 80
+    assert cfg.min_rel_improvement == pytest.approx(1e-3)
+    assert cfg.fixed_epochs == 0 and cfg.progress_interval_seconds == 0
+    assert lib.aco_is_significant_improvement(100.0, 99.95, 1e-3) == 0
+    assert lib.aco_is_significant_improvement(100.0, 99.8, 1e-3) == 1
+    assert lib.aco_is_significant_improvement(1.7976931348623157e308, 5.0, 1e-3) == 1
+
+
+def test_cli_arg_parsers():
+    lib = N.load()
+    v = C.c_int32()
+    assert lib.cli_parse_int_arg(b"42", C.byref(v)) == 1 and v.value == 42
+    for bad in (b"", b"-1", b"4x", b"100000001"):
+        assert lib.cli_parse_int_arg(bad, C.byref(v)) == 0
+    ok = C.c_int32()
+    assert lib.cli_parse_uint_arg(b"4294967295", C.byref(ok)) == 4294967295 and ok.value == 1
+    lib.cli_parse_uint_arg(b"4294967296", C.byref(ok))
+    assert ok.value == 0
+    assert lib.aco_status_string(3) == b"no solution"
+
+
+@pytest.mark.parametrize("argv,needle", [
+    ([], "usage"),
+    (["x.vrp", "2"], "usage"),
+    (["GOLDEN/ref_smoke.vrp", "-2", "0"], "usage"),
+    (["GOLDEN/ref_smoke.vrp", "3", "0", "1"], "VEHICLES mismatch"),
+    (["GOLDEN/missing.vrp", "2", "0"], "fopen"),
+])
+def test_cli_error_exits(argv, needle):
+    """Failure paths that end before any GPU call (main_vrp.cu:61-89)."""
+    argv = [a.replace("GOLDEN", GOLDEN) for a in argv]
+    r = subprocess.run([N.CLI_PATH] + argv, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert needle in r.stderr

@@ -1,0 +1,1008 @@
+/*
+ * aco_hip.hip - host side of the C ABI declared in include/aco_hip.h.
+ *
+ * Replaces the CUDA host driver aco_vrp_cuda_with_capacity
+ * (src/cuda/aco_cuda.cu:188-454): the context owns every device pool, every
+ * launch is checked, the iteration-best / global-best / stagnation logic runs
+ * on the device (no per-epoch device-to-host copy of ant summaries or routes),
+ * and epochs are launched in chunks, optionally replayed from a hipGraph.
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "aco_hip.h"
+#include "aco_host.h"
+#include "aco_inline.h"
+#include "aco_kernels.h"
+
+#define LDS_LIMIT 163840u
+#define HIST ACO_HIST
+
+struct TimingRec {
+  int kind;
+  hipEvent_t a, b;
+};
+
+struct AcoHipContext {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char err[512] = {0};
+  bool loaded = false;
+  AcoHipParams prm{};
+  int num_inst = 0, num_col = 0, W = 4, stage = 1, groups = 1, total_rows = 0;
+  size_t lds_bytes = 0;
+  std::vector<InstDesc> inst;
+  std::vector<ColonyDesc> col;
+  std::vector<int> inst_n1;
+  AcoTauTables tables{};
+  std::vector<float> eta_lut;
+  int eta_len = 0;
+  size_t pher_total = 0, tours_total = 0, best_total = 0, rec_total = 0, nodes_total = 0;
+  int ants_total = 0, max_stride = 0, max_pher_vec = 0;
+  /* device pools */
+  InstDesc *d_inst = nullptr;
+  ColonyDesc *d_col = nullptr;
+  AcoHipColonyState *d_state = nullptr;
+  int *d_row_inst = nullptr;
+  int2 *d_xy = nullptr;
+  uint16_t *d_dem = nullptr;
+  float *d_dscore = nullptr;
+  uint32_t *d_rec = nullptr;
+  uint8_t *d_pher = nullptr;
+  int32_t *d_tours = nullptr, *d_len = nullptr, *d_unvis = nullptr, *d_best = nullptr;
+  long long *d_cost = nullptr;
+  float *d_deq = nullptr, *d_thr = nullptr, *d_taupow = nullptr, *d_eta = nullptr;
+  int32_t *d_scratch_tour = nullptr;
+  HistEntry *d_hist = nullptr;
+  DevCtx P{};
+  /* timing */
+  bool timing = false;
+  std::vector<TimingRec> pending;
+  std::vector<hipEvent_t> free_events;
+  AcoHipTiming t{};
+  /* graph */
+  hipGraphExec_t graph = nullptr;
+  int graph_chunk = 0;
+};
+
+/* ---- helpers --------------------------------------------------------------- */
+
+static int set_err(AcoHipContext *c, int code, const char *fmt, ...) {
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPC(ctx, call)                                                                         \
+  do {                                                                                          \
+    hipError_t e__ = (call);                                                                    \
+    if (e__ != hipSuccess)                                                                      \
+      return set_err(ctx, ACO_HIP_ERR_BACKEND, "%s failed at %s:%d: %s", #call, __FILE__,      \
+                     __LINE__, hipGetErrorString(e__));                                         \
+  } while (0)
+
+#define LAUNCHED(ctx, name)                                                                     \
+  do {                                                                                          \
+    hipError_t e__ = hipGetLastError();                                                         \
+    if (e__ != hipSuccess)                                                                      \
+      return set_err(ctx, ACO_HIP_ERR_BACKEND, "launch of %s failed: %s", name,                 \
+                     hipGetErrorString(e__));                                                   \
+  } while (0)
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static size_t r16(size_t x) { return (x + 15) & ~(size_t)15; }
+static size_t r256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <typename T>
+static int dalloc(AcoHipContext *c, T **p, size_t bytes) {
+  bytes = r256(bytes ? bytes : 1) + 256;
+  HIPC(c, hipMalloc((void **)p, bytes));
+  HIPC(c, hipMemset(*p, 0, bytes));
+  return ACO_HIP_OK;
+}
+
+static void free_pools(AcoHipContext *c) {
+  void *ptrs[] = {c->d_inst, c->d_col,   c->d_state,  c->d_row_inst, c->d_xy,   c->d_dem,
+                  c->d_dscore, c->d_rec, c->d_pher,   c->d_tours,    c->d_len,  c->d_unvis,
+                  c->d_best, c->d_cost,  c->d_deq,    c->d_thr,      c->d_taupow, c->d_eta,
+                  c->d_scratch_tour, c->d_hist};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  c->d_inst = nullptr; c->d_col = nullptr; c->d_state = nullptr; c->d_row_inst = nullptr;
+  c->d_xy = nullptr; c->d_dem = nullptr; c->d_dscore = nullptr; c->d_rec = nullptr;
+  c->d_pher = nullptr; c->d_tours = nullptr; c->d_len = nullptr; c->d_unvis = nullptr;
+  c->d_best = nullptr; c->d_cost = nullptr; c->d_deq = nullptr; c->d_thr = nullptr;
+  c->d_taupow = nullptr; c->d_eta = nullptr; c->d_scratch_tour = nullptr; c->d_hist = nullptr;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  c->loaded = false;
+}
+
+/* LDS bytes of one construct workgroup for an instance of n1 nodes. */
+static size_t lds_for(int n1, int k, int eta_len, int W, int stage) {
+  size_t b = 256 * 4 + r16((size_t)eta_len * 4) + (size_t)W * r16((size_t)n1 * 2);
+  if (stage)
+    b += r16((size_t)n1 * k * 4) + r16((size_t)n1 * k) + r16((size_t)n1 * 4) + r16((size_t)n1 * 8);
+  return b;
+}
+
+/* ---- timing ---------------------------------------------------------------- */
+
+enum { T_CONSTRUCT, T_BEST, T_EVAP, T_DEPOSIT, T_KNN };
+
+static hipEvent_t get_event(AcoHipContext *c) {
+  if (!c->free_events.empty()) {
+    hipEvent_t e = c->free_events.back();
+    c->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static int resolve_timing(AcoHipContext *c) {
+  if (c->pending.empty()) return ACO_HIP_OK;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  for (TimingRec &r : c->pending) {
+    float ms = 0.0f;
+    HIPC(c, hipEventElapsedTime(&ms, r.a, r.b));
+    switch (r.kind) {
+      case T_CONSTRUCT: c->t.construct_ms += ms; c->t.construct_launches++; break;
+      case T_BEST: c->t.best_ms += ms; c->t.best_launches++; break;
+      case T_EVAP: c->t.evaporate_ms += ms; c->t.evaporate_launches++; break;
+      case T_DEPOSIT: c->t.deposit_ms += ms; c->t.deposit_launches++; break;
+      default: c->t.knn_ms += ms; c->t.knn_launches++; break;
+    }
+    c->free_events.push_back(r.a);
+    c->free_events.push_back(r.b);
+  }
+  c->pending.clear();
+  return ACO_HIP_OK;
+}
+
+struct Timed {
+  AcoHipContext *c;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(AcoHipContext *cc, int k) : c(cc), kind(k) {
+    if (c->timing) {
+      a = get_event(c);
+      b = get_event(c);
+      if (a && b) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~Timed() {
+    if (c->timing && a && b) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({kind, a, b});
+      if (c->pending.size() >= 4096) (void)resolve_timing(c);
+    }
+  }
+};
+
+/* ---- context ---------------------------------------------------------------- */
+
+extern "C" int aco_hip_create(AcoHipContext **out, int device) {
+  if (!out) return ACO_HIP_ERR_INVALID_INPUT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return ACO_HIP_ERR_BACKEND;
+  if (device < 0 || device >= count) return ACO_HIP_ERR_INVALID_INPUT;
+  AcoHipContext *c = new AcoHipContext();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return ACO_HIP_ERR_BACKEND;
+  }
+  *out = c;
+  return ACO_HIP_OK;
+}
+
+extern "C" void aco_hip_destroy(AcoHipContext *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (TimingRec &r : c->pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
+  free_pools(c);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char *aco_hip_last_error(const AcoHipContext *c) { return c ? c->err : "null context"; }
+
+extern "C" int aco_hip_sync(AcoHipContext *c) {
+  if (!c) return ACO_HIP_ERR_INVALID_INPUT;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ACO_HIP_OK;
+}
+
+static int env_int(const char *name, int dflt) {
+  const char *s = getenv(name);
+  return (s && *s) ? atoi(s) : dflt;
+}
+
+extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
+                            const AcoHipParams *p) {
+  if (!c || !in || num <= 0 || !p) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null argument");
+  if (p->cand_k < 1 || p->cand_k > 64)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "cand_k must be in [1, 64]");
+  if (p->ants < 1 || p->ants > (1 << 20))
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "ants must be in [1, 2^20]");
+  if (p->colonies < 1) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "colonies must be >= 1");
+  if (!(p->rho > 0.0f && p->rho < 1.0f))
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "rho must be in (0, 1)");
+  if (!(p->tau0 > 0.0f)) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tau0 must be > 0");
+  if (p->knn_key != ACO_HIP_KNN_EXACT && p->knn_key != ACO_HIP_KNN_REF_F32)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "unknown knn_key");
+  HIPC(c, hipSetDevice(c->device));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  free_pools(c);
+  c->prm = *p;
+  c->num_inst = num;
+  c->num_col = num * p->colonies;
+  c->inst.assign(num, InstDesc{});
+  c->inst_n1.assign(num, 0);
+  int dmax = 0;
+  size_t node_off = 0, cand_off = 0;
+  int row_base = 0;
+  for (int i = 0; i < num; ++i) {
+    const AcoHipInstance &a = in[i];
+    if (a.n < 1 || a.n > 65534 || a.K < 1 || a.capacity < 1 || a.capacity > 32767 || !a.x ||
+        !a.y || !a.demand)
+      return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
+                     "instance %d: need 1 <= n <= 65534, K >= 1, 1 <= capacity <= 32767", i);
+    if (a.demand[0] != 0)
+      return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "instance %d: depot demand must be 0", i);
+    int32_t xmin = a.x[0], xmax = a.x[0], ymin = a.y[0], ymax = a.y[0];
+    long long total = 0;
+    for (int j = 0; j <= a.n; ++j) {
+      if (a.x[j] < 0 || a.x[j] > 32767 || a.y[j] < 0 || a.y[j] > 32767)
+        return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
+                       "instance %d: coordinates must be integers in [0, 32767]", i);
+      if (a.demand[j] < 0 || a.demand[j] > 32767)
+        return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "instance %d: demand out of [0, 32767]", i);
+      total += j ? a.demand[j] : 0;
+      xmin = std::min(xmin, a.x[j]); xmax = std::max(xmax, a.x[j]);
+      ymin = std::min(ymin, a.y[j]); ymax = std::max(ymax, a.y[j]);
+    }
+    dmax = std::max(dmax, (int)aco_nint_dist(xmin, ymin, xmax, ymax));
+    InstDesc &d = c->inst[i];
+    d.n = a.n;
+    d.K = a.K;
+    d.k = std::min(p->cand_k, a.n);
+    d.Q = a.capacity;
+    d.node_off = (int)node_off;
+    d.row_base = row_base;
+    d.total_demand = total;
+    d.cand_off = (long long)cand_off;
+    d.seed = a.seed;
+    c->inst_n1[i] = a.n + 1;
+    node_off += ((size_t)a.n + 1 + 3) & ~(size_t)3;
+    cand_off += (((size_t)a.n + 1) * d.k + 3) & ~(size_t)3;
+    row_base += a.n + 1;
+  }
+  c->nodes_total = node_off;
+  c->rec_total = cand_off;
+  c->total_rows = row_base;
+
+  /* tables */
+  aco_tau_tables_build(&c->tables, p->alpha, p->rho, p->tau0);
+  c->eta_len = dmax + 1;
+  c->eta_lut.assign((size_t)c->eta_len, 0.0f);
+  aco_eta_table_build(c->eta_lut.data(), dmax, p->beta);
+
+  /* colonies */
+  c->col.assign(c->num_col, ColonyDesc{});
+  size_t pher = 0, tours = 0, best = 0;
+  int ants = 0, max_stride = 0;
+  size_t max_vec = 0;
+  for (int i = 0; i < num; ++i) {
+    const InstDesc &d = c->inst[i];
+    const size_t n1 = (size_t)d.n + 1;
+    const int stride = (int)((((size_t)d.n + d.K + 1) + 63) & ~(size_t)63);
+    for (int cc = 0; cc < p->colonies; ++cc) {
+      ColonyDesc &cd = c->col[(size_t)i * p->colonies + cc];
+      cd.inst = i;
+      cd.colony_in_inst = cc;
+      cd.m = p->ants;
+      cd.stride = stride;
+      cd.pher_off = (long long)pher;
+      cd.tcand_off = (long long)(pher + r16(n1 * n1));
+      cd.pher_bytes = (long long)(r16(n1 * n1) + r16(n1 * d.k));
+      cd.tour_off = (long long)tours;
+      cd.ant_off = ants;
+      cd.best_off = (int)best;
+      pher += r256((size_t)cd.pher_bytes);
+      tours += (size_t)p->ants * stride;
+      best += stride;
+      ants += p->ants;
+      max_stride = std::max(max_stride, stride);
+      max_vec = std::max(max_vec, (size_t)cd.pher_bytes / 16);
+    }
+  }
+  c->pher_total = pher;
+  c->tours_total = tours;
+  c->best_total = best;
+  c->ants_total = ants;
+  c->max_stride = max_stride;
+  c->max_pher_vec = (int)std::min(max_vec, (size_t)1 << 30);
+
+  /* launch shape: ants per workgroup and LDS staging */
+  int W = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
+  W = std::max(1, std::min(16, std::min(W, p->ants)));
+  int want_stage = p->stage_lds;
+  if (want_stage < 0) want_stage = env_int("ACO_HIP_STAGE_LDS", -1);
+  int max_n1 = 0, max_k = 0;
+  size_t need_stage = 0, need_plain = 0;
+  for (int i = 0; i < num; ++i) {
+    max_n1 = std::max(max_n1, c->inst_n1[i]);
+    max_k = std::max(max_k, c->inst[i].k);
+    need_stage = std::max(need_stage, lds_for(c->inst_n1[i], c->inst[i].k, c->eta_len, W, 1));
+    need_plain = std::max(need_plain, lds_for(c->inst_n1[i], c->inst[i].k, c->eta_len, W, 0));
+  }
+  int stage;
+  if (want_stage == 1) stage = 1;
+  else if (want_stage == 0) stage = 0;
+  else stage = need_stage <= LDS_LIMIT ? 1 : 0;
+  size_t need = stage ? need_stage : need_plain;
+  while (need > LDS_LIMIT && W > 1) {
+    --W;
+    need = 0;
+    for (int i = 0; i < num; ++i)
+      need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, c->eta_len, W, stage));
+  }
+  if (need > LDS_LIMIT)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
+                   "instance too large for the LDS layout (%zu bytes needed, stage=%d)", need,
+                   stage);
+  c->W = W;
+  c->stage = stage;
+  c->groups = (p->ants + W - 1) / W;
+  c->lds_bytes = need;
+
+  /* device pools */
+  int rc;
+  if ((rc = dalloc(c, &c->d_inst, sizeof(InstDesc) * num))) return rc;
+  if ((rc = dalloc(c, &c->d_col, sizeof(ColonyDesc) * c->num_col))) return rc;
+  if ((rc = dalloc(c, &c->d_state, sizeof(AcoHipColonyState) * c->num_col))) return rc;
+  if ((rc = dalloc(c, &c->d_hist, sizeof(HistEntry) * HIST * c->num_col))) return rc;
+  if ((rc = dalloc(c, &c->d_row_inst, sizeof(int) * (size_t)c->total_rows))) return rc;
+  if ((rc = dalloc(c, &c->d_xy, sizeof(int2) * c->nodes_total))) return rc;
+  if ((rc = dalloc(c, &c->d_dem, sizeof(uint16_t) * c->nodes_total))) return rc;
+  if ((rc = dalloc(c, &c->d_dscore, sizeof(float) * c->nodes_total))) return rc;
+  if ((rc = dalloc(c, &c->d_rec, sizeof(uint32_t) * c->rec_total))) return rc;
+  if ((rc = dalloc(c, &c->d_pher, c->pher_total))) return rc;
+  if ((rc = dalloc(c, &c->d_tours, sizeof(int32_t) * c->tours_total))) return rc;
+  if ((rc = dalloc(c, &c->d_len, sizeof(int32_t) * (size_t)ants))) return rc;
+  if ((rc = dalloc(c, &c->d_unvis, sizeof(int32_t) * (size_t)ants))) return rc;
+  if ((rc = dalloc(c, &c->d_cost, sizeof(long long) * (size_t)ants))) return rc;
+  if ((rc = dalloc(c, &c->d_best, sizeof(int32_t) * c->best_total))) return rc;
+  if ((rc = dalloc(c, &c->d_scratch_tour, sizeof(int32_t) * (size_t)max_stride))) return rc;
+  if ((rc = dalloc(c, &c->d_deq, sizeof(float) * 256))) return rc;
+  if ((rc = dalloc(c, &c->d_thr, sizeof(float) * 256))) return rc;
+  if ((rc = dalloc(c, &c->d_taupow, sizeof(float) * 256))) return rc;
+  if ((rc = dalloc(c, &c->d_eta, sizeof(float) * (size_t)c->eta_len))) return rc;
+
+  std::vector<int2> xy(c->nodes_total, make_int2(0, 0));
+  std::vector<uint16_t> dem(c->nodes_total, 0);
+  std::vector<int> row_inst((size_t)c->total_rows);
+  for (int i = 0; i < num; ++i) {
+    const InstDesc &d = c->inst[i];
+    for (int j = 0; j <= d.n; ++j) {
+      xy[(size_t)d.node_off + j] = make_int2(in[i].x[j], in[i].y[j]);
+      dem[(size_t)d.node_off + j] = (uint16_t)in[i].demand[j];
+      row_inst[(size_t)d.row_base + j] = i;
+    }
+  }
+  HIPC(c, hipMemcpy(c->d_inst, c->inst.data(), sizeof(InstDesc) * num, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_col, c->col.data(), sizeof(ColonyDesc) * c->num_col,
+                    hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_row_inst, row_inst.data(), sizeof(int) * row_inst.size(),
+                    hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_xy, xy.data(), sizeof(int2) * xy.size(), hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_dem, dem.data(), sizeof(uint16_t) * dem.size(), hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_deq, c->tables.deq, sizeof(float) * 256, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_thr, c->tables.thr, sizeof(float) * 255, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_taupow, c->tables.taupow, sizeof(float) * 256, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_eta, c->eta_lut.data(), sizeof(float) * c->eta_lut.size(),
+                    hipMemcpyHostToDevice));
+
+  DevCtx &P = c->P;
+  P.inst = c->d_inst;
+  P.col = c->d_col;
+  P.state = c->d_state;
+  P.row_inst = c->d_row_inst;
+  P.xy = c->d_xy;
+  P.dem = c->d_dem;
+  P.dscore = c->d_dscore;
+  P.rec = c->d_rec;
+  P.pher = c->d_pher;
+  P.tours = c->d_tours;
+  P.len = c->d_len;
+  P.cost = c->d_cost;
+  P.unvis = c->d_unvis;
+  P.best = c->d_best;
+  P.deq = c->d_deq;
+  P.thr = c->d_thr;
+  P.taupow = c->d_taupow;
+  P.eta = c->d_eta;
+  P.eta_len = c->eta_len;
+  P.q_evap = c->tables.q_evap;
+  P.qdep = p->Q;
+  P.fixed_epochs = 0;
+  P.stagnation = 0;
+  P.max_epochs = 0;
+  P.min_rel = 1e-3;
+  P.hist = (void *)c->d_hist;
+
+  HIPC(c, hipFuncSetAttribute((const void *)k_construct<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
+  HIPC(c, hipFuncSetAttribute((const void *)k_construct<false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
+  c->loaded = true;
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_num_colonies(const AcoHipContext *c) { return c ? c->num_col : 0; }
+
+extern "C" int aco_hip_get_shape(const AcoHipContext *c, int *cand_k, int *ants, int *stride,
+                                 int *apg, int *stage) {
+  if (!c || !c->loaded) return ACO_HIP_ERR_INVALID_INPUT;
+  if (cand_k) *cand_k = c->inst[0].k;
+  if (ants) *ants = c->prm.ants;
+  if (stride) *stride = c->col[0].stride;
+  if (apg) *apg = c->W;
+  if (stage) *stage = c->stage;
+  return ACO_HIP_OK;
+}
+
+#define NEED_LOADED(c)                                                                          \
+  do {                                                                                          \
+    if (!(c)) return ACO_HIP_ERR_INVALID_INPUT;                                                 \
+    if (!(c)->loaded) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "no instance loaded");       \
+  } while (0)
+
+/* ---- kernels ----------------------------------------------------------------- */
+
+extern "C" int aco_hip_build_candidates(AcoHipContext *c) {
+  NEED_LOADED(c);
+  const int blocks = (c->total_rows * 64 + 255) / 256;
+  {
+    Timed tm(c, T_KNN);
+    hipLaunchKernelGGL(k_knn, dim3(blocks), dim3(256), 0, c->stream, c->P, c->total_rows,
+                       c->prm.knn_key, c->prm.depot_close_weight);
+  }
+  LAUNCHED(c, "k_knn");
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_init_tau(AcoHipContext *c) {
+  NEED_LOADED(c);
+  const uint32_t q = c->tables.q_tau0;
+  const uint32_t word = q | (q << 8) | (q << 16) | (q << 24);
+  const size_t vecs = c->pher_total / 16;
+  const int blocks = (int)std::min<size_t>((vecs + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_tau, dim3(std::max(blocks, 1)), dim3(256), 0, c->stream, c->d_pher,
+                     c->pher_total, word);
+  LAUNCHED(c, "k_fill_tau");
+  std::vector<AcoHipColonyState> st(c->num_col);
+  for (AcoHipColonyState &s : st) {
+    memset(&s, 0, sizeof(s));
+    s.best_ant = -1;
+    s.iter_best_cost = INT64_MAX;
+    s.global_best_cost = INT64_MAX;
+  }
+  HIPC(c, hipMemcpyAsync(c->d_state, st.data(), sizeof(AcoHipColonyState) * st.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ACO_HIP_OK;
+}
+
+static int launch_construct(AcoHipContext *c) {
+  {
+    Timed tm(c, T_CONSTRUCT);
+    dim3 grid((unsigned)(c->num_col * c->groups)), block((unsigned)(64 * c->W));
+    if (c->stage)
+      hipLaunchKernelGGL(k_construct<true>, grid, block, c->lds_bytes, c->stream, c->P, c->groups,
+                         c->W);
+    else
+      hipLaunchKernelGGL(k_construct<false>, grid, block, c->lds_bytes, c->stream, c->P,
+                         c->groups, c->W);
+  }
+  LAUNCHED(c, "k_construct");
+  return ACO_HIP_OK;
+}
+
+static int launch_best(AcoHipContext *c) {
+  {
+    Timed tm(c, T_BEST);
+    hipLaunchKernelGGL(k_select_best, dim3(c->num_col), dim3(256), 0, c->stream, c->P);
+  }
+  LAUNCHED(c, "k_select_best");
+  return ACO_HIP_OK;
+}
+
+static int launch_evap(AcoHipContext *c) {
+  const int x = std::max(1, std::min((c->max_pher_vec + 255) / 256, 2048));
+  {
+    Timed tm(c, T_EVAP);
+    hipLaunchKernelGGL(k_evaporate, dim3(x, c->num_col), dim3(256), 0, c->stream, c->P);
+  }
+  LAUNCHED(c, "k_evaporate");
+  return ACO_HIP_OK;
+}
+
+static int launch_deposit(AcoHipContext *c, int which) {
+  {
+    Timed tm(c, T_DEPOSIT);
+    hipLaunchKernelGGL(k_deposit, dim3(c->num_col), dim3(1024), 0, c->stream, c->P, which);
+  }
+  LAUNCHED(c, "k_deposit");
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_construct(AcoHipContext *c) {
+  NEED_LOADED(c);
+  return launch_construct(c);
+}
+extern "C" int aco_hip_select_best(AcoHipContext *c) {
+  NEED_LOADED(c);
+  return launch_best(c);
+}
+extern "C" int aco_hip_evaporate(AcoHipContext *c) {
+  NEED_LOADED(c);
+  return launch_evap(c);
+}
+extern "C" int aco_hip_deposit_iter(AcoHipContext *c) {
+  NEED_LOADED(c);
+  return launch_deposit(c, 0);
+}
+extern "C" int aco_hip_deposit_best(AcoHipContext *c) {
+  NEED_LOADED(c);
+  return launch_deposit(c, 1);
+}
+
+static int launch_epochs(AcoHipContext *c, int count) {
+  int rc;
+  for (int e = 0; e < count; ++e) {
+    if ((rc = launch_construct(c))) return rc;
+    if ((rc = launch_best(c))) return rc;
+    if ((rc = launch_evap(c))) return rc;
+    if ((rc = launch_deposit(c, 0))) return rc;
+    if ((rc = launch_deposit(c, 1))) return rc;
+  }
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_epochs(AcoHipContext *c, int count) {
+  NEED_LOADED(c);
+  if (count < 0) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "negative epoch count");
+  return launch_epochs(c, count);
+}
+
+/* ---- epoch loop ---------------------------------------------------------------- */
+
+static int capture_chunk(AcoHipContext *c, int chunk) {
+  if (c->graph && c->graph_chunk == chunk) return ACO_HIP_OK;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const bool timing = c->timing;
+  c->timing = false;
+  int rc = launch_epochs(c, chunk);
+  c->timing = timing;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return set_err(c, ACO_HIP_ERR_BACKEND, "hipStreamEndCapture: %s",
+                                      hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return set_err(c, ACO_HIP_ERR_BACKEND, "hipGraphInstantiate: %s",
+                                      hipGetErrorString(e));
+  c->graph_chunk = chunk;
+  return ACO_HIP_OK;
+}
+
+static void print_progress(int epoch, double elapsed, double max_rt, int since, int max_stag,
+                           long long best) {
+  char tr[32], er[32], bc[64];
+  if (max_rt > 0.0) snprintf(tr, sizeof(tr), "%.1fs", std::max(0.0, max_rt - elapsed));
+  else snprintf(tr, sizeof(tr), "senza limite");
+  if (max_stag > 0) {
+    int rem = max_stag - since;
+    snprintf(er, sizeof(er), "%d", rem > 0 ? rem : 0);
+    if (rem <= 0) snprintf(tr, sizeof(tr), "0.0s");
+  } else {
+    snprintf(er, sizeof(er), "senza limite");
+  }
+  if (best != INT64_MAX) snprintf(bc, sizeof(bc), "%.3f", (double)best);
+  else snprintf(bc, sizeof(bc), "non disponibile");
+  printf("[progress][hip] epoca=%d | epoche_rimanenti_prima_stop=%s | tempo_trascorso=%.1fs | "
+         "tempo_rimanente=%s | best_cost=%s\n",
+         epoch, er, elapsed, tr, bc);
+  fflush(stdout);
+}
+
+extern "C" int aco_hip_run(AcoHipContext *c, const AcoHipRunConfig *cfg, AcoHipRunStats *stats) {
+  NEED_LOADED(c);
+  if (!cfg) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null run config");
+  HIPC(c, hipSetDevice(c->device));
+  c->P.fixed_epochs = cfg->fixed_epochs > 0 ? cfg->fixed_epochs : 0;
+  c->P.stagnation = cfg->stagnation_epochs > 0 ? cfg->stagnation_epochs : 0;
+  c->P.max_epochs = cfg->max_epochs > 0 ? cfg->max_epochs : 0;
+  c->P.min_rel = cfg->min_rel_improvement > 0.0 ? cfg->min_rel_improvement : 1e-3;
+  if (!c->P.fixed_epochs && !c->P.stagnation && !c->P.max_epochs && !(cfg->timeout_seconds > 0))
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
+                   "no stop rule: set a timeout, stagnation, fixed or max epochs");
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  int chunk = cfg->chunk_epochs > 0 ? std::min(cfg->chunk_epochs, HIST) : 8;
+  const bool use_graph = cfg->use_graph && !c->timing;
+  std::vector<AcoHipColonyState> st(c->num_col);
+  std::vector<HistEntry> hist(HIST);
+  HIPC(c, hipMemcpy(st.data(), c->d_state, sizeof(AcoHipColonyState) * c->num_col,
+                    hipMemcpyDeviceToHost));
+  const int start_epoch = st[0].epoch;
+  if (cfg->print_progress) {
+    printf("HIP Solver starting... (N=%d, K=%d, M=%d)\n", c->inst[0].n, c->inst[0].K,
+           c->prm.ants);
+    fflush(stdout);
+  }
+  const double t0 = now_s();
+  double next_beat = cfg->progress_interval_seconds > 0 ? t0 + cfg->progress_interval_seconds : 0;
+  int printed_upto = start_epoch;
+  int rc;
+  for (;;) {
+    double t = now_s();
+    if (cfg->timeout_seconds > 0.0 && (t - t0) > cfg->timeout_seconds) break;
+    if (use_graph) {
+      if ((rc = capture_chunk(c, chunk))) return rc;
+      HIPC(c, hipGraphLaunch(c->graph, c->stream));
+    } else {
+      if ((rc = launch_epochs(c, chunk))) return rc;
+    }
+    HIPC(c, hipMemcpyAsync(st.data(), c->d_state, sizeof(AcoHipColonyState) * c->num_col,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(hist.data(), c->d_hist, sizeof(HistEntry) * HIST,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const double now = now_s();
+    const double elapsed = now - t0;
+    if (cfg->print_progress) {
+      for (int e = printed_upto + 1; e <= st[0].epoch; ++e) {
+        const HistEntry &h = hist[(e - 1) % HIST];
+        if (h.epoch != e) continue;
+        const int reached_time = cfg->timeout_seconds > 0.0 && elapsed >= cfg->timeout_seconds;
+        const int reached_stag = h.halted;
+        if (e == 1 || e % 10 == 0 || h.improved || reached_time || reached_stag)
+          print_progress(e, elapsed, cfg->timeout_seconds, h.iter_since_best,
+                         cfg->stagnation_epochs, h.global_best);
+      }
+      printed_upto = st[0].epoch;
+    }
+    if (next_beat > 0 && now >= next_beat) {
+      fprintf(stderr, "[hip] elapsed %.1fs, iter %d, best %.3f\n", elapsed, st[0].epoch,
+              st[0].global_best_cost == INT64_MAX ? 0.0 : (double)st[0].global_best_cost);
+      next_beat = now + cfg->progress_interval_seconds;
+    }
+    bool all_halted = true;
+    for (const AcoHipColonyState &s : st) all_halted = all_halted && s.halted;
+    if (all_halted) break;
+  }
+  const double secs = now_s() - t0;
+  if (stats) {
+    stats->epochs = st[0].epoch - start_epoch;
+    stats->seconds = secs;
+    long long tours = 0;
+    for (const AcoHipColonyState &s : st) tours += (long long)(s.epoch - start_epoch) * c->prm.ants;
+    stats->ant_tours_per_s = secs > 0 ? (double)tours / secs : 0.0;
+  }
+  return resolve_timing(c);
+}
+
+/* ---- downloads / uploads ------------------------------------------------------ */
+
+static int check_inst(AcoHipContext *c, int i) {
+  if (i < 0 || i >= c->num_inst) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "bad instance %d", i);
+  return ACO_HIP_OK;
+}
+static int check_col(AcoHipContext *c, int i) {
+  if (i < 0 || i >= c->num_col) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "bad colony %d", i);
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_candidates(AcoHipContext *c, int inst, int32_t *idx, int32_t *dist,
+                                      float *eta) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_inst(c, inst))) return rc;
+  const InstDesc &d = c->inst[inst];
+  const size_t cnt = (size_t)(d.n + 1) * d.k;
+  std::vector<uint32_t> rec(cnt);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(rec.data(), c->d_rec + d.cand_off, cnt * 4, hipMemcpyDeviceToHost));
+  for (size_t e = 0; e < cnt; ++e) {
+    const uint32_t j = rec[e] & 0xFFFFu, dd = rec[e] >> 16;
+    if (idx) idx[e] = (int32_t)j;
+    if (dist) dist[e] = (int32_t)dd;
+    if (eta) eta[e] = j ? c->eta_lut[dd] : 0.0f;
+  }
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_set_candidate_eta(AcoHipContext *c, int inst, const float *eta) {
+  NEED_LOADED(c);
+  (void)inst;
+  (void)eta;
+  return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "learned candidate heuristic not enabled");
+}
+
+extern "C" int aco_hip_get_tau(AcoHipContext *c, int colony, uint8_t *tau, uint8_t *tau_cand) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_col(c, colony))) return rc;
+  const ColonyDesc &cd = c->col[colony];
+  const InstDesc &d = c->inst[cd.inst];
+  const size_t n1 = (size_t)d.n + 1;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (tau) HIPC(c, hipMemcpy(tau, c->d_pher + cd.pher_off, n1 * n1, hipMemcpyDeviceToHost));
+  if (tau_cand)
+    HIPC(c, hipMemcpy(tau_cand, c->d_pher + cd.tcand_off, n1 * d.k, hipMemcpyDeviceToHost));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_set_tau(AcoHipContext *c, int colony, const uint8_t *tau) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_col(c, colony))) return rc;
+  if (!tau) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null tau");
+  const ColonyDesc &cd = c->col[colony];
+  const InstDesc &d = c->inst[cd.inst];
+  const size_t n1 = (size_t)d.n + 1, cnt = n1 * d.k;
+  std::vector<uint32_t> rec(cnt);
+  std::vector<uint8_t> tc(cnt);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(rec.data(), c->d_rec + d.cand_off, cnt * 4, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n1; ++i)
+    for (int t = 0; t < d.k; ++t) tc[i * d.k + t] = tau[i * n1 + (rec[i * d.k + t] & 0xFFFFu)];
+  HIPC(c, hipMemcpy(c->d_pher + cd.pher_off, tau, n1 * n1, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->d_pher + cd.tcand_off, tc.data(), cnt, hipMemcpyHostToDevice));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_tours(AcoHipContext *c, int colony, int32_t *tours, int32_t *len,
+                                 int64_t *cost, int32_t *unvisited) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_col(c, colony))) return rc;
+  const ColonyDesc &cd = c->col[colony];
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (tours)
+    HIPC(c, hipMemcpy(tours, c->d_tours + cd.tour_off, sizeof(int32_t) * (size_t)cd.m * cd.stride,
+                      hipMemcpyDeviceToHost));
+  if (len)
+    HIPC(c, hipMemcpy(len, c->d_len + cd.ant_off, sizeof(int32_t) * cd.m, hipMemcpyDeviceToHost));
+  if (cost)
+    HIPC(c, hipMemcpy(cost, c->d_cost + cd.ant_off, sizeof(int64_t) * cd.m, hipMemcpyDeviceToHost));
+  if (unvisited)
+    HIPC(c, hipMemcpy(unvisited, c->d_unvis + cd.ant_off, sizeof(int32_t) * cd.m,
+                      hipMemcpyDeviceToHost));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_colony(AcoHipContext *c, int colony, AcoHipColonyState *st) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_col(c, colony))) return rc;
+  if (!st) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null state");
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(st, c->d_state + colony, sizeof(*st), hipMemcpyDeviceToHost));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_set_epoch(AcoHipContext *c, int colony, int epoch) {
+  AcoHipColonyState st;
+  int rc = aco_hip_get_colony(c, colony, &st);
+  if (rc) return rc;
+  st.epoch = epoch;
+  HIPC(c, hipMemcpy(c->d_state + colony, &st, sizeof(st), hipMemcpyHostToDevice));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_best(AcoHipContext *c, int colony, int32_t *tour, int32_t *len,
+                                int64_t *cost) {
+  AcoHipColonyState st;
+  int rc = aco_hip_get_colony(c, colony, &st);
+  if (rc) return rc;
+  if (st.global_best_cost == INT64_MAX)
+    return set_err(c, ACO_HIP_ERR_NO_SOLUTION, "no feasible tour found");
+  if (tour)
+    HIPC(c, hipMemcpy(tour, c->d_best + c->col[colony].best_off, sizeof(int32_t) * st.best_len,
+                      hipMemcpyDeviceToHost));
+  if (len) *len = st.best_len;
+  if (cost) *cost = st.global_best_cost;
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_deposit_tour(AcoHipContext *c, int colony, const int32_t *tour, int len,
+                                    float delta) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_col(c, colony))) return rc;
+  const ColonyDesc &cd = c->col[colony];
+  const InstDesc &d = c->inst[cd.inst];
+  if (!tour || len < 2 || len > cd.stride)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tour length must be in [2, %d]", cd.stride);
+  for (int p = 0; p < len; ++p)
+    if (tour[p] < 0 || tour[p] > d.n)
+      return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tour node out of range at %d", p);
+  HIPC(c, hipMemcpyAsync(c->d_scratch_tour, tour, sizeof(int32_t) * len, hipMemcpyHostToDevice,
+                         c->stream));
+  hipLaunchKernelGGL(k_deposit_tour, dim3(1), dim3(1024), 0, c->stream, c->P, colony,
+                     c->d_scratch_tour, len, delta);
+  LAUNCHED(c, "k_deposit_tour");
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_route_costs(AcoHipContext *c, int inst, const int32_t *tours,
+                                   const int32_t *len, int count, int stride, int64_t *cost) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_inst(c, inst))) return rc;
+  if (count <= 0) return ACO_HIP_OK;
+  if (!tours || !len || !cost || stride <= 0)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "bad route-cost arguments");
+  const int n = c->inst[inst].n;
+  for (int t = 0; t < count; ++t) {
+    if (len[t] < 0 || len[t] > stride)
+      return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tour %d length out of range", t);
+    for (int p = 0; p < len[t]; ++p)
+      if (tours[(size_t)t * stride + p] < 0 || tours[(size_t)t * stride + p] > n)
+        return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tour %d node out of range", t);
+  }
+  int32_t *dt = nullptr, *dl = nullptr;
+  long long *dc = nullptr;
+  HIPC(c, hipMalloc(&dt, sizeof(int32_t) * (size_t)count * stride));
+  HIPC(c, hipMalloc(&dl, sizeof(int32_t) * count));
+  HIPC(c, hipMalloc(&dc, sizeof(long long) * count));
+  HIPC(c, hipMemcpy(dt, tours, sizeof(int32_t) * (size_t)count * stride, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(dl, len, sizeof(int32_t) * count, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_route_costs, dim3((count * 64 + 255) / 256), dim3(256), 0, c->stream, c->P,
+                     inst, dt, dl, count, stride, dc);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(cost, dc, sizeof(long long) * count, hipMemcpyDeviceToHost);
+  (void)hipFree(dt);
+  (void)hipFree(dl);
+  (void)hipFree(dc);
+  if (e != hipSuccess)
+    return set_err(c, ACO_HIP_ERR_BACKEND, "route costs: %s", hipGetErrorString(e));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_tables(AcoHipContext *c, float *deq, float *thr, float *taupow,
+                                  float *eta_lut, int *eta_len, uint8_t *q_tau0, uint8_t *q_evap) {
+  NEED_LOADED(c);
+  if (deq) memcpy(deq, c->tables.deq, sizeof(float) * 256);
+  if (thr) memcpy(thr, c->tables.thr, sizeof(float) * 255);
+  if (taupow) memcpy(taupow, c->tables.taupow, sizeof(float) * 256);
+  if (eta_lut) memcpy(eta_lut, c->eta_lut.data(), sizeof(float) * c->eta_lut.size());
+  if (eta_len) *eta_len = c->eta_len;
+  if (q_tau0) *q_tau0 = c->tables.q_tau0;
+  if (q_evap) *q_evap = c->tables.q_evap;
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_set_timing(AcoHipContext *c, int enable) {
+  if (!c) return ACO_HIP_ERR_INVALID_INPUT;
+  int rc = resolve_timing(c);
+  if (rc) return rc;
+  c->timing = enable != 0;
+  memset(&c->t, 0, sizeof(c->t));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_timing(AcoHipContext *c, AcoHipTiming *t) {
+  if (!c || !t) return ACO_HIP_ERR_INVALID_INPUT;
+  int rc = resolve_timing(c);
+  if (rc) return rc;
+  *t = c->t;
+  return ACO_HIP_OK;
+}
+
+/* ---- one-call solver ------------------------------------------------------------ */
+
+extern "C" int aco_vrp_hip_with_capacity(int n, int K, int capacity, int m, const int32_t *x,
+                                         const int32_t *y, const int32_t *demand, double alpha,
+                                         double beta, double rho, double tau0, double Q,
+                                         unsigned int seed, int32_t *best_tour, int32_t *best_len,
+                                         int64_t *best_cost) {
+  if (n <= 0 || K <= 0 || !x || !y || !demand || !best_tour || !best_len || !best_cost)
+    return ACO_HIP_ERR_INVALID_INPUT;
+  AcoRuntimeConfig rt;
+  aco_runtime_config_load_env(&rt);
+  if (m == 0) m = 256; /* aco_cuda.cu:206 */
+  AcoHipContext *c = nullptr;
+  int rc = aco_hip_create(&c, env_int("ACO_HIP_DEVICE", 0));
+  if (rc) {
+    fprintf(stderr, "HIP device unavailable\n");
+    return rc;
+  }
+  AcoHipInstance in{n, K, capacity, x, y, demand, seed};
+  AcoHipParams p{};
+  p.cand_k = std::min(env_int("ACO_HIP_CAND_K", 32), n); /* aco_cuda.cu:198-199 */
+  p.ants = m;
+  p.colonies = 1;
+  p.alpha = (float)alpha;
+  p.beta = (float)beta;
+  p.rho = (float)rho;
+  p.tau0 = (float)tau0;
+  p.Q = (float)Q;
+  p.depot_close_weight = 2.0f;
+  p.knn_key = env_int("ACO_HIP_KNN_KEY", ACO_HIP_KNN_EXACT);
+  p.ants_per_group = env_int("ACO_HIP_ANTS_PER_GROUP", 0);
+  p.stage_lds = env_int("ACO_HIP_STAGE_LDS", -1);
+  if ((rc = aco_hip_load(c, &in, 1, &p)) || (rc = aco_hip_build_candidates(c)) ||
+      (rc = aco_hip_init_tau(c))) {
+    fprintf(stderr, "HIP solver setup failed: %s\n", aco_hip_last_error(c));
+    aco_hip_destroy(c);
+    return rc;
+  }
+  AcoHipRunConfig cfg{};
+  cfg.timeout_seconds = rt.timeout_seconds;
+  cfg.stagnation_epochs = rt.stagnation_epochs;
+  cfg.min_rel_improvement = rt.min_rel_improvement;
+  cfg.fixed_epochs = rt.fixed_epochs;
+  cfg.max_epochs = env_int("ACO_HIP_MAX_EPOCHS", 0);
+  if (!cfg.fixed_epochs && !cfg.stagnation_epochs && !(cfg.timeout_seconds > 0) &&
+      !cfg.max_epochs)
+    cfg.stagnation_epochs = 100; /* bounded default when no stop rule is configured */
+  cfg.progress_interval_seconds = rt.progress_interval_seconds;
+  cfg.print_progress = 1;
+  cfg.chunk_epochs = env_int("ACO_HIP_CHUNK", 8);
+  cfg.use_graph = env_int("ACO_HIP_GRAPH", 1);
+  AcoHipRunStats stats{};
+  rc = aco_hip_run(c, &cfg, &stats);
+  if (rc) {
+    fprintf(stderr, "HIP solver failed: %s\n", aco_hip_last_error(c));
+    aco_hip_destroy(c);
+    return rc;
+  }
+  rc = aco_hip_get_best(c, 0, best_tour, best_len, best_cost);
+  aco_hip_destroy(c);
+  return rc;
+}

@@ -73,6 +73,16 @@ struct DevCtx {
   int stagnation;
   int max_epochs;
   double min_rel;
+  void *hist;          /* HistEntry[colonies][ACO_HIST] per-epoch progress ring */
+};
+
+#define ACO_HIST 64
+struct HistEntry {
+  int epoch;
+  int iter_since_best;
+  int improved;
+  int halted;
+  long long global_best;
 };
 
 static inline __host__ __device__ size_t round16(size_t x) { return (x + 15) & ~(size_t)15; }

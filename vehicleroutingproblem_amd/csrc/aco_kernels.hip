@@ -495,6 +495,13 @@ __global__ void __launch_bounds__(256) k_select_best(DevCtx P) {
     if (P.fixed_epochs > 0) S->halted = S->epoch >= P.fixed_epochs;
     else if (P.stagnation > 0 && S->iter_since_best >= P.stagnation) S->halted = 1;
     if (P.max_epochs > 0 && S->epoch >= P.max_epochs) S->halted = 1;
+    HistEntry *h = reinterpret_cast<HistEntry *>(P.hist) + (size_t)colony * ACO_HIST +
+                   (S->epoch - 1) % ACO_HIST;
+    h->epoch = S->epoch;
+    h->iter_since_best = S->iter_since_best;
+    h->improved = S->improved;
+    h->halted = S->halted;
+    h->global_best = S->global_best_cost;
   }
   __syncthreads();
   const int ant = s_copy;

@@ -54,6 +54,10 @@ void aco_eta_table_build(float *eta, int dmax, float beta) {
   }
 }
 
+void aco_philox_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  aco_philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
+}
+
 uint8_t aco_tau_quantize(const AcoTauTables *t, float tau) {
   return (uint8_t)aco_quantize_thr(t->thr, tau);
 }
