@@ -41,7 +41,7 @@ def cpu_baseline(inst, seconds_budget=12.0):
     o.epochs(1, threads)  # warm caches
     t0 = time.perf_counter()
     epochs = 0
-    while time.perf_counter() - t0 < seconds_budget and epochs < 200:
+    while time.perf_counter() - t0 < seconds_budget and epochs < 5000:
         o.epochs(1, threads)
         epochs += 1
     dt = time.perf_counter() - t0
@@ -126,12 +126,19 @@ def main():
     value = ANTS * epochs / dt
 
     # per-kernel durations: hipEvents on the context stream around each launch
+    # (eager launches; the same epochs continue the colony)
     prof_epochs = min(args.steps, 50)
     s.set_timing(True)
-    s.epochs(prof_epochs)
+    s.run(fixed_epochs=st["epoch"] + prof_epochs, chunk_epochs=8, use_graph=False)
     s.sync()
     t = s.timing()
     s.set_timing(False)
+    assert t["construct_launches"] >= prof_epochs, t
+    # device counters over a few more epochs
+    s.set_stats(True)
+    s.run(fixed_epochs=s.colony_state()["epoch"] + 10, chunk_epochs=10, use_graph=True)
+    dev = s.stats()
+    s.set_stats(False)
     construct_ms = t["construct_ms"] / max(1, t["construct_launches"])
     evap_ms = t["evaporate_ms"] / max(1, t["evaporate_launches"])
     k = min(CAND_K, inst.n)
@@ -162,8 +169,13 @@ def main():
                      "avg_launch_ms": round(construct_ms, 5),
                      "bytes_per_launch": bytes_per_launch,
                      "evaporate_avg_ms": round(evap_ms, 5)},
-        "kernel_ms_per_epoch": {key: round(t[key + "_ms"] / max(1, prof_epochs), 5)
+        "kernel_ms_per_epoch": {key: round(t[key + "_ms"] / max(1, t["construct_launches"]), 5)
                                 for key in ("construct", "best", "evaporate", "deposit")},
+        "construct_counters": {
+            "decisions_per_tour": round(dev["decisions"] / max(1, dev["ants"]), 1),
+            "fallback_rate": round(dev["fallbacks"] / max(1, dev["decisions"]), 4),
+            "fallback_cycle_share": round(dev["fallback_cycles"] / max(1, dev["ant_cycles"]), 4),
+            "cycles_per_decision": round(dev["ant_cycles"] / max(1, dev["decisions"]), 1)},
     }
     s.close()
     if not args.no_cpu:

@@ -207,6 +207,22 @@ int aco_hip_route_costs(AcoHipContext *ctx, int instance, const int32_t *tours,
 int aco_hip_get_tables(AcoHipContext *ctx, float *deq, float *thr, float *taupow,
                        float *eta_lut, int *eta_len, uint8_t *q_tau0, uint8_t *q_evap);
 
+/* Device-side construction counters (the reference declares CudaIterStats,
+ * include/aco_cuda_kernels.h:63-74, but never fills it). */
+typedef struct {
+  uint64_t decisions;        /* roulette decisions (moves and route closes) */
+  uint64_t fallbacks;        /* decisions that needed the full-row fallback */
+  uint64_t fallback_cycles;  /* shader clocks spent in the fallback */
+  uint64_t ant_cycles;       /* shader clocks from ant start to tour end */
+  uint64_t ants;             /* tours counted */
+  uint64_t staging_cycles;   /* shader clocks per workgroup spent staging LDS, summed */
+} AcoHipStats;
+
+/* Enable (1) / disable (0) the counters; enabling clears them. */
+int aco_hip_set_stats(AcoHipContext *ctx, int enable);
+/* Counters summed since enabling, for one colony. */
+int aco_hip_get_stats(AcoHipContext *ctx, int colony, AcoHipStats *st);
+
 /* ---- timing ------------------------------------------------------------- */
 int aco_hip_set_timing(AcoHipContext *ctx, int enable);
 int aco_hip_get_timing(AcoHipContext *ctx, AcoHipTiming *t);

@@ -186,6 +186,22 @@ def test_run_stagnation_matches_oracle(gpu):
     s.close()
 
 
+def test_consecutive_runs_continue(gpu):
+    """A second run with a later fixed-epoch target continues the colony
+    (warmup then timed region, as bench.py does)."""
+    inst, s, o = make_pair("c1_n100")
+    s.prepare()
+    assert s.run(fixed_epochs=5, chunk_epochs=4)["epochs"] == 5
+    assert s.run(fixed_epochs=12, chunk_epochs=8)["epochs"] == 7
+    assert s.run(fixed_epochs=12, chunk_epochs=8)["epochs"] == 0
+    o.set_stop_rules(fixed=12)
+    o.epochs(12)
+    assert s.colony_state()["epoch"] == 12
+    assert s.colony_state()["global_best_cost"] == o.state()["global_best_cost"]
+    assert np.array_equal(s.tau()[0], o.tau()[0])
+    s.close()
+
+
 def test_batch_matches_single(gpu):
     """Instances batched in one context with two colonies each give the
     tours of each instance run alone (the Philox stream depends only on the

@@ -68,6 +68,12 @@ class AcoHipTiming(C.Structure):
                 ("knn_ms", C.c_double), ("knn_launches", C.c_int)]
 
 
+class AcoHipStats(C.Structure):
+    _fields_ = [("decisions", C.c_uint64), ("fallbacks", C.c_uint64),
+                ("fallback_cycles", C.c_uint64), ("ant_cycles", C.c_uint64),
+                ("ants", C.c_uint64), ("staging_cycles", C.c_uint64)]
+
+
 class VrpInstance(C.Structure):
     _fields_ = [("n", C.c_int), ("K", C.c_int), ("capacity", C.c_int),
                 ("x", i32p), ("y", i32p), ("demand", i32p), ("unit_demand", C.c_int),
@@ -113,6 +119,8 @@ HIP_SYMBOLS = {
     "aco_hip_deposit_tour": (C.c_int, [ctx_p, C.c_int, i32p, C.c_int, C.c_float]),
     "aco_hip_route_costs": (C.c_int, [ctx_p, C.c_int, i32p, i32p, C.c_int, C.c_int, i64p]),
     "aco_hip_get_tables": (C.c_int, [ctx_p, f32p, f32p, f32p, f32p, i32p, u8p, u8p]),
+    "aco_hip_set_stats": (C.c_int, [ctx_p, C.c_int]),
+    "aco_hip_get_stats": (C.c_int, [ctx_p, C.c_int, C.c_void_p]),
     "aco_hip_set_timing": (C.c_int, [ctx_p, C.c_int]),
     "aco_hip_get_timing": (C.c_int, [ctx_p, C.POINTER(AcoHipTiming)]),
     "aco_vrp_hip_with_capacity": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, i32p, i32p, i32p,

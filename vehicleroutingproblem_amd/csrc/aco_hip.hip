@@ -62,6 +62,7 @@ struct AcoHipContext {
   float *d_deq = nullptr, *d_thr = nullptr, *d_taupow = nullptr, *d_eta = nullptr;
   int32_t *d_scratch_tour = nullptr;
   HistEntry *d_hist = nullptr;
+  unsigned long long *d_stats = nullptr;
   DevCtx P{};
   /* timing */
   bool timing = false;
@@ -122,7 +123,7 @@ static void free_pools(AcoHipContext *c) {
   void *ptrs[] = {c->d_inst, c->d_col,   c->d_state,  c->d_row_inst, c->d_xy,   c->d_dem,
                   c->d_dscore, c->d_rec, c->d_pher,   c->d_tours,    c->d_len,  c->d_unvis,
                   c->d_best, c->d_cost,  c->d_deq,    c->d_thr,      c->d_taupow, c->d_eta,
-                  c->d_scratch_tour, c->d_hist};
+                  c->d_scratch_tour, c->d_hist, c->d_stats};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_inst = nullptr; c->d_col = nullptr; c->d_state = nullptr; c->d_row_inst = nullptr;
@@ -130,6 +131,7 @@ static void free_pools(AcoHipContext *c) {
   c->d_pher = nullptr; c->d_tours = nullptr; c->d_len = nullptr; c->d_unvis = nullptr;
   c->d_best = nullptr; c->d_cost = nullptr; c->d_deq = nullptr; c->d_thr = nullptr;
   c->d_taupow = nullptr; c->d_eta = nullptr; c->d_scratch_tour = nullptr; c->d_hist = nullptr;
+  c->d_stats = nullptr;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -354,7 +356,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
 
   /* launch shape: ants per workgroup and LDS staging */
   int W = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
-  W = std::max(1, std::min(16, std::min(W, p->ants)));
+  W = std::max(1, std::min(8, std::min(W, p->ants)));
   int want_stage = p->stage_lds;
   if (want_stage < 0) want_stage = env_int("ACO_HIP_STAGE_LDS", -1);
   int max_n1 = 0, max_k = 0;
@@ -391,6 +393,8 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if ((rc = dalloc(c, &c->d_col, sizeof(ColonyDesc) * c->num_col))) return rc;
   if ((rc = dalloc(c, &c->d_state, sizeof(AcoHipColonyState) * c->num_col))) return rc;
   if ((rc = dalloc(c, &c->d_hist, sizeof(HistEntry) * HIST * c->num_col))) return rc;
+  if ((rc = dalloc(c, &c->d_stats, sizeof(unsigned long long) * ACO_STATS * c->num_col)))
+    return rc;
   if ((rc = dalloc(c, &c->d_row_inst, sizeof(int) * (size_t)c->total_rows))) return rc;
   if ((rc = dalloc(c, &c->d_xy, sizeof(int2) * c->nodes_total))) return rc;
   if ((rc = dalloc(c, &c->d_dem, sizeof(uint16_t) * c->nodes_total))) return rc;
@@ -459,6 +463,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.max_epochs = 0;
   P.min_rel = 1e-3;
   P.hist = (void *)c->d_hist;
+  P.stats = nullptr;
 
   HIPC(c, hipFuncSetAttribute((const void *)k_construct<true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
@@ -673,8 +678,17 @@ extern "C" int aco_hip_run(AcoHipContext *c, const AcoHipRunConfig *cfg, AcoHipR
   const bool use_graph = cfg->use_graph && !c->timing;
   std::vector<AcoHipColonyState> st(c->num_col);
   std::vector<HistEntry> hist(HIST);
+  HIPC(c, hipStreamSynchronize(c->stream));
   HIPC(c, hipMemcpy(st.data(), c->d_state, sizeof(AcoHipColonyState) * c->num_col,
                     hipMemcpyDeviceToHost));
+  /* a new run re-evaluates the halted flag under its own stop rules */
+  for (AcoHipColonyState &s : st) {
+    if (c->P.fixed_epochs > 0) s.halted = s.epoch >= c->P.fixed_epochs;
+    else s.halted = c->P.stagnation > 0 && s.iter_since_best >= c->P.stagnation;
+    if (c->P.max_epochs > 0 && s.epoch >= c->P.max_epochs) s.halted = 1;
+  }
+  HIPC(c, hipMemcpy(c->d_state, st.data(), sizeof(AcoHipColonyState) * c->num_col,
+                    hipMemcpyHostToDevice));
   const int start_epoch = st[0].epoch;
   if (cfg->print_progress) {
     printf("HIP Solver starting... (N=%d, K=%d, M=%d)\n", c->inst[0].n, c->inst[0].K,
@@ -685,7 +699,9 @@ extern "C" int aco_hip_run(AcoHipContext *c, const AcoHipRunConfig *cfg, AcoHipR
   double next_beat = cfg->progress_interval_seconds > 0 ? t0 + cfg->progress_interval_seconds : 0;
   int printed_upto = start_epoch;
   int rc;
-  for (;;) {
+  bool halted0 = true;
+  for (const AcoHipColonyState &s : st) halted0 = halted0 && s.halted;
+  for (; !halted0;) {
     double t = now_s();
     if (cfg->timeout_seconds > 0.0 && (t - t0) > cfg->timeout_seconds) break;
     if (use_graph) {
@@ -924,6 +940,35 @@ extern "C" int aco_hip_get_tables(AcoHipContext *c, float *deq, float *thr, floa
   if (eta_len) *eta_len = c->eta_len;
   if (q_tau0) *q_tau0 = c->tables.q_tau0;
   if (q_evap) *q_evap = c->tables.q_evap;
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_set_stats(AcoHipContext *c, int enable) {
+  NEED_LOADED(c);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemset(c->d_stats, 0, sizeof(unsigned long long) * ACO_STATS * c->num_col));
+  c->P.stats = enable ? c->d_stats : nullptr;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_stats(AcoHipContext *c, int colony, AcoHipStats *st) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_col(c, colony))) return rc;
+  if (!st) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null stats");
+  unsigned long long v[ACO_STATS];
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(v, c->d_stats + (size_t)colony * ACO_STATS, sizeof(v), hipMemcpyDeviceToHost));
+  st->decisions = v[0];
+  st->fallbacks = v[1];
+  st->fallback_cycles = v[2];
+  st->ant_cycles = v[3];
+  st->ants = v[4];
+  st->staging_cycles = v[5];
   return ACO_HIP_OK;
 }
 

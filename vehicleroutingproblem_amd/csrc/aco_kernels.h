@@ -74,7 +74,12 @@ struct DevCtx {
   int max_epochs;
   double min_rel;
   void *hist;          /* HistEntry[colonies][ACO_HIST] per-epoch progress ring */
+  unsigned long long *stats; /* [colonies][ACO_STATS] counters, NULL = off */
 };
+
+/* construct counters per colony: decisions, fallbacks, fallback cycles,
+ * ant cycles, ants (clock64 = shader clock) */
+#define ACO_STATS 8
 
 #define ACO_HIST 64
 struct HistEntry {

@@ -48,9 +48,17 @@ __device__ __forceinline__ float wave_scan_rows(float v, int lane, int rows) {
   v = v + f_dpp_row_shr2(v);
   v = v + f_dpp_row_shr4(v);
   v = v + f_dpp_row_shr8(v);
-  for (int r = 1; r < rows; ++r) {
-    float c = f_readlane(v, 16 * r - 1);
-    if ((lane >> 4) == r) v = v + c;
+  if (rows > 1) {
+    const float c = f_readlane(v, 15);
+    if ((lane >> 4) == 1) v = v + c;
+  }
+  if (rows > 2) {
+    const float c = f_readlane(v, 31);
+    if ((lane >> 4) == 2) v = v + c;
+  }
+  if (rows > 3) {
+    const float c = f_readlane(v, 47);
+    if ((lane >> 4) == 3) v = v + c;
   }
   return v;
 }
@@ -143,54 +151,73 @@ __global__ void __launch_bounds__(256) k_fill_tau(uint8_t *pher, size_t bytes, u
 
 /* ---- tour construction --------------------------------------------------- */
 
-struct Lds {
-  const uint32_t *rec; /* (n+1)*k records idx | d<<16 */
-  const uint8_t *tc;   /* (n+1)*k candidate pheromone */
-  const float *taupow; /* 256 */
-  const float *eta;    /* eta_len */
-  const float *dscore; /* n+1 */
-  const int2 *xy;      /* n+1 */
-  volatile uint16_t *state; /* this ant's n+1 visited/demand states */
-};
-
 /* Copies `bytes` (multiple of 16, 16-byte aligned on both sides) with the
  * whole workgroup. */
 __device__ __forceinline__ void wg_copy16(void *dst, const void *src, size_t bytes) {
-  const size_t nvec = bytes / 16;
+  const int nvec = (int)(bytes / 16);
+  const int nt = (int)blockDim.x;
   uint4 *d = reinterpret_cast<uint4 *>(dst);
   const uint4 *s = reinterpret_cast<const uint4 *>(src);
-  for (size_t e = threadIdx.x; e < nvec; e += blockDim.x) d[e] = s[e];
+  /* eight loads in flight per thread before the LDS stores */
+  int e = (int)threadIdx.x;
+  for (; e + 7 * nt < nvec; e += 8 * nt) {
+    uint4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = s[e + u * nt];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) d[e + u * nt] = v[u];
+  }
+  for (; e < nvec; e += nt) d[e] = s[e];
+}
+
+struct FbOut {
+  int move;
+  int mdem;
+  float T;
+};
+
+/* Weights of the 16 customers j0..j0+15 of one lane (0 when not eligible);
+ * every load is issued before any is used. */
+__device__ __forceinline__ float fb_lane_weights(float (&w)[FB_V], int j0, int n, int room,
+                                                 int2 pc, const int2 *xy, const float *taupow,
+                                                 const float *eta, const uint16_t *state,
+                                                 const uint8_t *trow) {
+  uint32_t st[FB_V], tq[FB_V];
+  int2 pj[FB_V];
+#pragma unroll
+  for (int i = 0; i < FB_V; ++i) {
+    const int j = j0 + i;
+    const int jc = j <= n ? j : n;
+    st[i] = state[jc];
+    tq[i] = trow[jc];
+    pj[i] = xy[jc];
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < FB_V; ++i) {
+    const int d = aco_nint_dist(pc.x, pc.y, pj[i].x, pj[i].y);
+    const float v = taupow[tq[i]] * eta[d];
+    w[i] = (j0 + i <= n && (int)st[i] <= room) ? v : 0.0f;
+    s = s + w[i];
+  }
+  return s;
 }
 
 /* Fallback roulette over every unvisited customer that fits (the exact
  * two-pass restatement of select_global_fallback, aco_cuda_kernels.cu:336-430).
- * Customer j = 1 + c*1024 + lane*16 + i.  Returns the node or -1, the total
- * weight in T and the node's demand in mdem. */
-__device__ __noinline__ int fallback_select(const Lds L, const uint8_t *__restrict__ trow, int n,
-                                            int cur, int room, float u, float &T, int &mdem,
-                                            int lane) {
+ * Customer j = 1 + c*1024 + lane*16 + i.  Returns the node (or -1), its demand
+ * and the total weight. */
+__device__ __forceinline__ FbOut fallback_select(const int2 *xy, const float *taupow,
+                                                 const float *eta, const uint16_t *state,
+                                                 const uint8_t *__restrict__ trow, int n, int cur,
+                                                 int room, float u, int lane) {
   const int nch = (n + FB_CHUNK - 1) / FB_CHUNK;
-  const int2 pc = L.xy[cur];
+  const int2 pc = xy[cur];
   float w[FB_V];
   float Rv = 0.0f, Cv = 0.0f, R = 0.0f, s = 0.0f, Pl = 0.0f;
   for (int c = 0; c < nch; ++c) {
-    const int j0 = 1 + c * FB_CHUNK + lane * FB_V;
-    s = 0.0f;
-#pragma unroll
-    for (int i = 0; i < FB_V; ++i) {
-      const int j = j0 + i;
-      const int jc = j <= n ? j : n;
-      const uint32_t st = L.state[jc];
-      const uint8_t tq = trow[jc];
-      float wt = 0.0f;
-      if (j <= n && (int)st <= room) {
-        const int2 pj = L.xy[jc];
-        const int d = aco_nint_dist(pc.x, pc.y, pj.x, pj.y);
-        wt = L.taupow[tq] * L.eta[d];
-      }
-      w[i] = wt;
-      s = s + wt;
-    }
+    s = fb_lane_weights(w, 1 + c * FB_CHUNK + lane * FB_V, n, room, pc, xy, taupow, eta, state,
+                        trow);
     Pl = wave_scan_rows(s, lane, 4);
     const float Cc = f_readlane(Pl, 63);
     R = R + Cc;
@@ -199,34 +226,22 @@ __device__ __noinline__ int fallback_select(const Lds L, const uint8_t *__restri
       Cv = Cc;
     }
   }
-  T = R;
-  if (!(R > 0.0f)) return -1;
+  FbOut out;
+  out.T = R;
+  out.move = -1;
+  out.mdem = 0;
+  if (!(R > 0.0f)) return out;
   const float thr = u * R;
-  uint64_t m = ballot64(lane < nch && Rv > thr);
+  const uint64_t m = ballot64(lane < nch && Rv > thr);
   const int cs = m ? first_lane(m) : last_lane(ballot64(lane < nch && Cv > 0.0f));
   const float base = cs > 0 ? f_readlane(Rv, cs - 1) : 0.0f;
   const float thr_c = thr - base;
   if (cs != nch - 1) {
-    const int j0 = 1 + cs * FB_CHUNK + lane * FB_V;
-    s = 0.0f;
-#pragma unroll
-    for (int i = 0; i < FB_V; ++i) {
-      const int j = j0 + i;
-      const int jc = j <= n ? j : n;
-      const uint32_t st = L.state[jc];
-      const uint8_t tq = trow[jc];
-      float wt = 0.0f;
-      if (j <= n && (int)st <= room) {
-        const int2 pj = L.xy[jc];
-        const int d = aco_nint_dist(pc.x, pc.y, pj.x, pj.y);
-        wt = L.taupow[tq] * L.eta[d];
-      }
-      w[i] = wt;
-      s = s + wt;
-    }
+    s = fb_lane_weights(w, 1 + cs * FB_CHUNK + lane * FB_V, n, room, pc, xy, taupow, eta, state,
+                        trow);
     Pl = wave_scan_rows(s, lane, 4);
   }
-  uint64_t lm = ballot64(s > 0.0f && Pl > thr_c);
+  const uint64_t lm = ballot64(s > 0.0f && Pl > thr_c);
   const int ls = lm ? first_lane(lm) : last_lane(ballot64(s > 0.0f));
   const float pprev = ls > 0 ? f_readlane(Pl, ls - 1) : 0.0f;
   const float thr_l = thr_c - pprev;
@@ -242,13 +257,13 @@ __device__ __noinline__ int fallback_select(const Lds L, const uint8_t *__restri
   }
   if (is < 0) is = il;
   const int isel = __builtin_amdgcn_readlane(is, ls);
-  const int move = 1 + cs * FB_CHUNK + ls * FB_V + isel;
-  mdem = (int)L.state[move];
-  return move;
+  out.move = 1 + cs * FB_CHUNK + ls * FB_V + isel;
+  out.mdem = __builtin_amdgcn_readfirstlane((int)state[out.move]);
+  return out;
 }
 
 template <bool STAGE>
-__global__ void __launch_bounds__(1024) k_construct(DevCtx P, int groups_per_colony, int W) {
+__global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colony, int W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int colony = blockIdx.x / groups_per_colony;
   const int group = blockIdx.x - colony * groups_per_colony;
@@ -262,8 +277,12 @@ __global__ void __launch_bounds__(1024) k_construct(DevCtx P, int groups_per_col
   const int wave = threadIdx.x >> 6;
 
   /* ---- LDS carve-up and staging ---- */
+  const unsigned long long t_kernel = P.stats ? clock64() : 0ull;
   size_t off = 0;
-  Lds L;
+  const uint32_t *c_rec;
+  const uint8_t *c_tc;
+  const float *c_dsc;
+  const int2 *c_xy;
   if (STAGE) {
     uint32_t *s_rec = reinterpret_cast<uint32_t *>(smem + off);
     off += round16((size_t)n1 * k * 4);
@@ -277,15 +296,15 @@ __global__ void __launch_bounds__(1024) k_construct(DevCtx P, int groups_per_col
     wg_copy16(s_tc, P.pher + D.tcand_off, round16((size_t)n1 * k));
     wg_copy16(s_dsc, P.dscore + I.node_off, round16((size_t)n1 * 4));
     wg_copy16(s_xy, P.xy + I.node_off, round16((size_t)n1 * 8));
-    L.rec = s_rec;
-    L.tc = s_tc;
-    L.dscore = s_dsc;
-    L.xy = s_xy;
+    c_rec = s_rec;
+    c_tc = s_tc;
+    c_dsc = s_dsc;
+    c_xy = s_xy;
   } else {
-    L.rec = P.rec + I.cand_off;
-    L.tc = P.pher + D.tcand_off;
-    L.dscore = P.dscore + I.node_off;
-    L.xy = P.xy + I.node_off;
+    c_rec = P.rec + I.cand_off;
+    c_tc = P.pher + D.tcand_off;
+    c_dsc = P.dscore + I.node_off;
+    c_xy = P.xy + I.node_off;
   }
   float *s_taupow = reinterpret_cast<float *>(smem + off);
   off += 256 * 4;
@@ -293,29 +312,33 @@ __global__ void __launch_bounds__(1024) k_construct(DevCtx P, int groups_per_col
   off += round16((size_t)P.eta_len * 4);
   wg_copy16(s_taupow, P.taupow, 256 * 4);
   wg_copy16(s_eta, P.eta, round16((size_t)P.eta_len * 4));
-  L.taupow = s_taupow;
-  L.eta = s_eta;
   const size_t state_stride = round16((size_t)n1 * 2);
   uint16_t *s_state = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * state_stride);
   __syncthreads();
+  if (P.stats && threadIdx.x == 0)
+    atomicAdd(P.stats + (size_t)colony * ACO_STATS + 5, clock64() - t_kernel);
 
   const int ant = group * W + wave;
   if (ant >= D.m) return;
 
-  /* ---- reset (kernel_reset_ant_state, aco_cuda_kernels.cu:227-257) ---- */
+  /* ---- reset (kernel_reset_ant_state, aco_cuda_kernels.cu:227-257): the
+   * state of node j is its demand while unvisited, VISITED after; the depot
+   * starts visited ---- */
   const uint16_t *g_dem = P.dem + I.node_off;
   for (int j = lane; j < n1; j += 64) s_state[j] = (j == 0) ? (uint16_t)VISITED : g_dem[j];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  L.state = s_state;
 
   const uint8_t *__restrict__ tau = P.pher + D.pher_off;
   int32_t *tour_out = P.tours + D.tour_off + (size_t)ant * D.stride;
   const uint32_t key0 = I.seed;
   const uint32_t cid = (uint32_t)D.colony_in_inst;
   const int rows = (k + 15) >> 4;
+  const int klane = lane < k ? lane : 0;
 
+  const unsigned long long t_start = P.stats ? clock64() : 0ull;
+  unsigned long long fallbacks = 0, fb_cycles = 0;
   int rem_cnt = n;
   long long rem_dem = I.total_demand;
   uint32_t s = 0;
@@ -332,7 +355,7 @@ __global__ void __launch_bounds__(1024) k_construct(DevCtx P, int groups_per_col
     long long dd = 0;
     if (lane < cnt) {
       if (base + lane >= 1) {
-        const int2 a = L.xy[prev], b = L.xy[e];
+        const int2 a = c_xy[prev], b = c_xy[e];
         dd = aco_nint_dist(a.x, a.y, b.x, b.y);
       }
       tour_out[base + lane] = e;
@@ -364,18 +387,15 @@ __global__ void __launch_bounds__(1024) k_construct(DevCtx P, int groups_per_col
       const uint32_t u1 = (uint32_t)__builtin_amdgcn_readlane((int)rng1, (int)(s & 63u));
       ++s;
 
-      /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322) */
-      int j = 0;
-      uint32_t st = VISITED;
-      float w = 0.0f;
-      if (lane < k) {
-        const size_t e = (size_t)cur * k + lane;
-        const uint32_t rc = L.rec[e];
-        const uint8_t tq = L.tc[e];
-        j = (int)(rc & 0xFFFFu);
-        st = L.state[j];
-        if ((int)st <= room) w = L.taupow[tq] * L.eta[rc >> 16];
-      }
+      /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322):
+       * lanes >= k read slot 0 and are masked out */
+      const size_t e = (size_t)cur * k + klane;
+      const uint32_t rc = c_rec[e];
+      const uint32_t tq = c_tc[e];
+      const int j = lane < k ? (int)(rc & 0xFFFFu) : 0;
+      const uint32_t st = s_state[j];
+      const float v = s_taupow[tq] * s_eta[rc >> 16];
+      const float w = ((int)st <= room) ? v : 0.0f;
       const float Pf = wave_scan_rows(w, lane, rows);
       float T = f_readlane(Pf, k - 1);
       int move, mdem = 0;
@@ -386,19 +406,29 @@ __global__ void __launch_bounds__(1024) k_construct(DevCtx P, int groups_per_col
         move = __builtin_amdgcn_readlane(j, sel);
         mdem = __builtin_amdgcn_readlane((int)st, sel);
       } else {
-        move = fallback_select(L, tau + (size_t)cur * n1, n, cur, room, aco_u01(u0), T, mdem,
-                               lane);
+        const unsigned long long f0 = P.stats ? clock64() : 0ull;
+        const FbOut fb = fallback_select(c_xy, s_taupow, s_eta, s_state, tau + (size_t)cur * n1,
+                                         n, cur, room, aco_u01(u0), lane);
+        move = fb.move;
+        mdem = fb.mdem;
+        T = fb.T;
+        if (P.stats) {
+          fb_cycles += clock64() - f0;
+          ++fallbacks;
+        }
       }
       /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand */
       const bool close_allowed = rem_dem <= (long long)(K - veh - 1) * Q;
-      const float dsc = close_allowed ? L.dscore[cur] : 0.0f;
+      const float dsc =
+          close_allowed ? __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(c_dsc[cur])))
+                        : 0.0f;
       if (move > 0 && dsc > 0.0f) {
         const float thr2 = aco_u01(u1) * (T + dsc);
         if (thr2 >= T) move = 0;
       }
       if (move <= 0) break;
       append(move);
-      if (lane == 0) L.state[move] = (uint16_t)VISITED;
+      if (lane == 0) s_state[move] = (uint16_t)VISITED;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -415,6 +445,14 @@ __global__ void __launch_bounds__(1024) k_construct(DevCtx P, int groups_per_col
     P.len[a] = pos + 1;
     P.cost[a] = cost;
     P.unvis[a] = rem_cnt;
+    if (P.stats) {
+      unsigned long long *sc = P.stats + (size_t)colony * ACO_STATS;
+      atomicAdd(sc + 0, (unsigned long long)s);
+      atomicAdd(sc + 1, fallbacks);
+      atomicAdd(sc + 2, fb_cycles);
+      atomicAdd(sc + 3, clock64() - t_start);
+      atomicAdd(sc + 4, 1ull);
+    }
   }
 }
 

@@ -337,6 +337,14 @@ class HipSolver:
         return {"deq": deq, "thr": thr, "taupow": tp, "eta": eta, "q_tau0": q0.value,
                 "q_evap": qe.value}
 
+    def set_stats(self, enable: bool):
+        self._check(self.lib.aco_hip_set_stats(self._ctx, int(enable)))
+
+    def stats(self, colony: int = 0) -> dict:
+        st = N.AcoHipStats()
+        self._check(self.lib.aco_hip_get_stats(self._ctx, colony, C.byref(st)))
+        return {f: int(getattr(st, f)) for f, _ in st._fields_}
+
     def set_timing(self, enable: bool):
         self._check(self.lib.aco_hip_set_timing(self._ctx, int(enable)))
 
