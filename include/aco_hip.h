@@ -130,7 +130,7 @@ typedef struct {
 /* Per-kernel timing from hipEvents recorded on the context stream. */
 typedef struct {
   double construct_ms;  int construct_launches;
-  double best_ms;       int best_launches;
+  double best_ms;       int best_launches;      /* update kernel incl. fused deposits */
   double evaporate_ms;  int evaporate_launches;
   double deposit_ms;    int deposit_launches;
   double knn_ms;        int knn_launches;
@@ -173,7 +173,11 @@ int aco_hip_evaporate(AcoHipContext *ctx);
 int aco_hip_deposit_iter(AcoHipContext *ctx);
 /* Global-best deposit: replaces launch_deposit_flat_solution (aco_cuda_kernels.h:181). */
 int aco_hip_deposit_best(AcoHipContext *ctx);
-/* `count` full epochs: construct, select_best, evaporate, deposit_iter, deposit_best. */
+/* `count` full epochs, two launches each: construct, then one pheromone
+ * update launch whose blocks evaporate (deciding from the ants whether this
+ * epoch found a feasible tour) and whose last block to finish runs
+ * select_best, deposit_iter and deposit_best.  Same result as the reference
+ * order (select, evaporate, deposit, deposit). */
 int aco_hip_epochs(AcoHipContext *ctx, int count);
 
 /* ---- epoch loop: replaces aco_vrp_cuda_with_capacity (aco_cuda.cu:188-454) - */
@@ -188,6 +192,9 @@ int aco_hip_set_candidate_eta(AcoHipContext *ctx, int instance, const float *eta
 /* (n+1)^2 pheromone bytes and (n+1)*k candidate-aligned copy (either may be NULL). */
 int aco_hip_get_tau(AcoHipContext *ctx, int colony, uint8_t *tau, uint8_t *tau_cand);
 int aco_hip_set_tau(AcoHipContext *ctx, int colony, const uint8_t *tau);
+/* (n+1)*k candidate scores taupow[tau_cand] * eta^beta, the weights the
+ * construction samples (the reference OpenMP score matrix, aco_parallel.c:535-540). */
+int aco_hip_get_scores(AcoHipContext *ctx, int colony, float *score);
 /* tours: ants*tour_stride int32; len, unvisited: ants int32; cost: ants int64. */
 int aco_hip_get_tours(AcoHipContext *ctx, int colony, int32_t *tours, int32_t *len,
                       int64_t *cost, int32_t *unvisited);

@@ -52,18 +52,23 @@ static inline ACO_HD void aco_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
 /* Uniform float in [0,1) from the top 24 bits (exact conversion). */
 static inline ACO_HD float aco_u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
 
-/* Exact floor(sqrt(s)) for s < 2^32 - 2^17: float estimate, then integer fix-up. */
-static inline ACO_HD uint32_t aco_isqrt_u32(uint32_t s) {
-  uint32_t r = (uint32_t)__builtin_sqrtf((float)s);
-  if ((uint64_t)r * r > s) --r;
-  if ((uint64_t)(r + 1) * (r + 1) <= s) ++r;
-  return r;
+/* nint(sqrt(s)) for s <= 2*32767^2, exact.  A float estimate rounded to the
+ * nearest integer is within one of the answer; one integer test fixes it:
+ * nint(sqrt(s)) = r  <=>  r^2 - r < s <= r^2 + r. */
+static inline ACO_HD float aco_sqrt_estimate(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x); /* one v_sqrt_f32 (1 ulp): the fix-up below is exact */
+#else
+  return __builtin_sqrtf(x);
+#endif
 }
 
-/* nint(sqrt(s)): sqrt(s) >= r + 1/2  <=>  s > r^2 + r for integers. */
 static inline ACO_HD int32_t aco_nint_sqrt_u32(uint32_t s) {
-  uint32_t r = aco_isqrt_u32(s);
-  return (int32_t)(r + (s > r * r + r ? 1u : 0u));
+  uint32_t r = (uint32_t)(aco_sqrt_estimate((float)s) + 0.5f);
+  const uint32_t sq = r * r;
+  if (s > sq + r) ++r;
+  else if (r > 0u && s <= sq - r) --r;
+  return (int32_t)r;
 }
 
 /* TSPLIB EUC_2D distance for coordinates in [0, 32767]. */

@@ -32,6 +32,9 @@ CASES = {
     "tiny_n5": (lambda: Instance.uniform(5, 2, seed=3), dict(cand_k=20, ants=16, rho=0.5)),
     "n1030_two_chunks": (lambda: Instance.uniform(1030, 9, seed=11),
                          dict(cand_k=20, ants=64, rho=0.1)),
+    # full coordinate range: exercises the nint square root near 2*32767^2
+    "wide_coords": (lambda: Instance.uniform(200, 4, side=32767, seed=5),
+                    dict(cand_k=24, ants=32, rho=0.3)),
 }
 
 
@@ -117,6 +120,11 @@ def test_epochs_match_oracle(gpu, name):
         to, co = o.tau()
         assert np.array_equal(tg, to), f"tau differs after epoch {e}"
         assert np.array_equal(cg, co), f"tau_cand differs after epoch {e}"
+        # the score cache equals taupow[tau_cand] * eta[d] (one fp32 multiply)
+        t = o.tables()
+        _, dist, _ = o.candidates()
+        want = t["taupow"][co] * t["eta"][dist]
+        assert np.array_equal(s.scores().view(np.uint32), want.astype(np.float32).view(np.uint32))
     if o.best()[1] is not None:
         bt, bc = s.best()
         ot, oc = o.best()

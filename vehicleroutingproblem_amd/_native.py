@@ -112,6 +112,7 @@ HIP_SYMBOLS = {
     "aco_hip_set_candidate_eta": (C.c_int, [ctx_p, C.c_int, f32p]),
     "aco_hip_get_tau": (C.c_int, [ctx_p, C.c_int, u8p, u8p]),
     "aco_hip_set_tau": (C.c_int, [ctx_p, C.c_int, u8p]),
+    "aco_hip_get_scores": (C.c_int, [ctx_p, C.c_int, f32p]),
     "aco_hip_get_tours": (C.c_int, [ctx_p, C.c_int, i32p, i32p, i64p, i32p]),
     "aco_hip_get_colony": (C.c_int, [ctx_p, C.c_int, C.POINTER(AcoHipColonyState)]),
     "aco_hip_set_epoch": (C.c_int, [ctx_p, C.c_int, C.c_int]),

@@ -37,7 +37,7 @@ struct AcoHipContext {
   char err[512] = {0};
   bool loaded = false;
   AcoHipParams prm{};
-  int num_inst = 0, num_col = 0, W = 4, stage = 1, groups = 1, total_rows = 0;
+  int num_inst = 0, num_col = 0, W = 4, stage = 1, groups = 1, total_rows = 0, eta_lds = 1;
   size_t lds_bytes = 0;
   std::vector<InstDesc> inst;
   std::vector<ColonyDesc> col;
@@ -46,7 +46,8 @@ struct AcoHipContext {
   std::vector<float> eta_lut;
   int eta_len = 0;
   size_t pher_total = 0, tours_total = 0, best_total = 0, rec_total = 0, nodes_total = 0;
-  int ants_total = 0, max_stride = 0, max_pher_vec = 0;
+  size_t score_total = 0, slot_total = 0;
+  int ants_total = 0, max_stride = 0, max_mat_vec = 0, max_slots = 0;
   /* device pools */
   InstDesc *d_inst = nullptr;
   ColonyDesc *d_col = nullptr;
@@ -56,6 +57,9 @@ struct AcoHipContext {
   uint16_t *d_dem = nullptr;
   float *d_dscore = nullptr;
   uint32_t *d_rec = nullptr;
+  uint16_t *d_cidx = nullptr;
+  uint8_t *d_slot = nullptr;
+  float *d_score = nullptr;
   uint8_t *d_pher = nullptr;
   int32_t *d_tours = nullptr, *d_len = nullptr, *d_unvis = nullptr, *d_best = nullptr;
   long long *d_cost = nullptr;
@@ -63,6 +67,7 @@ struct AcoHipContext {
   int32_t *d_scratch_tour = nullptr;
   HistEntry *d_hist = nullptr;
   unsigned long long *d_stats = nullptr;
+  int *d_done = nullptr;
   DevCtx P{};
   /* timing */
   bool timing = false;
@@ -123,7 +128,7 @@ static void free_pools(AcoHipContext *c) {
   void *ptrs[] = {c->d_inst, c->d_col,   c->d_state,  c->d_row_inst, c->d_xy,   c->d_dem,
                   c->d_dscore, c->d_rec, c->d_pher,   c->d_tours,    c->d_len,  c->d_unvis,
                   c->d_best, c->d_cost,  c->d_deq,    c->d_thr,      c->d_taupow, c->d_eta,
-                  c->d_scratch_tour, c->d_hist, c->d_stats};
+                  c->d_scratch_tour, c->d_hist, c->d_stats, c->d_cidx, c->d_score, c->d_done, c->d_slot};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_inst = nullptr; c->d_col = nullptr; c->d_state = nullptr; c->d_row_inst = nullptr;
@@ -131,7 +136,8 @@ static void free_pools(AcoHipContext *c) {
   c->d_pher = nullptr; c->d_tours = nullptr; c->d_len = nullptr; c->d_unvis = nullptr;
   c->d_best = nullptr; c->d_cost = nullptr; c->d_deq = nullptr; c->d_thr = nullptr;
   c->d_taupow = nullptr; c->d_eta = nullptr; c->d_scratch_tour = nullptr; c->d_hist = nullptr;
-  c->d_stats = nullptr;
+  c->d_stats = nullptr; c->d_cidx = nullptr; c->d_score = nullptr; c->d_done = nullptr;
+  c->d_slot = nullptr;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -142,10 +148,14 @@ static void free_pools(AcoHipContext *c) {
 /* LDS bytes of one construct workgroup for an instance of n1 nodes. */
 static size_t lds_for(int n1, int k, int eta_len, int W, int stage) {
   size_t b = 256 * 4 + r16((size_t)eta_len * 4) + (size_t)W * r16((size_t)n1 * 2);
+  if (eta_len == 0) b = 256 * 4 + (size_t)W * r16((size_t)n1 * 2); /* eta table in global */
   if (stage)
-    b += r16((size_t)n1 * k * 4) + r16((size_t)n1 * k) + r16((size_t)n1 * 4) + r16((size_t)n1 * 8);
+    b += r16((size_t)n1 * k * 2) + r16((size_t)n1 * k * 4) + r16((size_t)n1 * 4) +
+         r16((size_t)n1 * 8);
   return b;
 }
+
+static int launch_refresh(AcoHipContext *c);
 
 /* ---- timing ---------------------------------------------------------------- */
 
@@ -270,7 +280,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   c->inst.assign(num, InstDesc{});
   c->inst_n1.assign(num, 0);
   int dmax = 0;
-  size_t node_off = 0, cand_off = 0;
+  size_t node_off = 0, cand_off = 0, slot_off = 0;
   int row_base = 0;
   for (int i = 0; i < num; ++i) {
     const AcoHipInstance &a = in[i];
@@ -302,13 +312,16 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     d.row_base = row_base;
     d.total_demand = total;
     d.cand_off = (long long)cand_off;
+    d.slot_off = (long long)slot_off;
+    slot_off += r256(((size_t)a.n + 1) * ((size_t)a.n + 1));
     d.seed = a.seed;
     c->inst_n1[i] = a.n + 1;
     node_off += ((size_t)a.n + 1 + 3) & ~(size_t)3;
-    cand_off += (((size_t)a.n + 1) * d.k + 3) & ~(size_t)3;
+    cand_off += (((size_t)a.n + 1) * d.k + 7) & ~(size_t)7;
     row_base += a.n + 1;
   }
   c->nodes_total = node_off;
+  c->slot_total = slot_off;
   c->rec_total = cand_off;
   c->total_rows = row_base;
 
@@ -320,9 +333,9 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
 
   /* colonies */
   c->col.assign(c->num_col, ColonyDesc{});
-  size_t pher = 0, tours = 0, best = 0;
+  size_t pher = 0, tours = 0, best = 0, score = 0;
   int ants = 0, max_stride = 0;
-  size_t max_vec = 0;
+  size_t max_vec = 0, max_slots = 0;
   for (int i = 0; i < num; ++i) {
     const InstDesc &d = c->inst[i];
     const size_t n1 = (size_t)d.n + 1;
@@ -339,12 +352,16 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       cd.tour_off = (long long)tours;
       cd.ant_off = ants;
       cd.best_off = (int)best;
+      cd.score_off = (long long)score;
+      cd.mat_bytes = (long long)r16(n1 * n1);
+      score += (n1 * d.k + 3) & ~(size_t)3;
+      max_slots = std::max(max_slots, n1 * d.k);
       pher += r256((size_t)cd.pher_bytes);
       tours += (size_t)p->ants * stride;
       best += stride;
       ants += p->ants;
       max_stride = std::max(max_stride, stride);
-      max_vec = std::max(max_vec, (size_t)cd.pher_bytes / 16);
+      max_vec = std::max(max_vec, (size_t)cd.mat_bytes / 16);
     }
   }
   c->pher_total = pher;
@@ -352,7 +369,9 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   c->best_total = best;
   c->ants_total = ants;
   c->max_stride = max_stride;
-  c->max_pher_vec = (int)std::min(max_vec, (size_t)1 << 30);
+  c->max_mat_vec = (int)std::min(max_vec, (size_t)1 << 30);
+  c->max_slots = (int)std::min(max_slots, (size_t)1 << 30);
+  c->score_total = score;
 
   /* launch shape: ants per workgroup and LDS staging */
   int W = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
@@ -367,16 +386,36 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     need_stage = std::max(need_stage, lds_for(c->inst_n1[i], c->inst[i].k, c->eta_len, W, 1));
     need_plain = std::max(need_plain, lds_for(c->inst_n1[i], c->inst[i].k, c->eta_len, W, 0));
   }
-  int stage;
-  if (want_stage == 1) stage = 1;
-  else if (want_stage == 0) stage = 0;
-  else stage = need_stage <= LDS_LIMIT ? 1 : 0;
-  size_t need = stage ? need_stage : need_plain;
-  while (need > LDS_LIMIT && W > 1) {
-    --W;
+  (void)need_stage;
+  (void)need_plain;
+  int stage = want_stage == 0 ? 0 : 1;
+  int eta_in_lds = 1;
+  size_t need = 0;
+  auto recompute = [&]() {
     need = 0;
     for (int i = 0; i < num; ++i)
-      need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, c->eta_len, W, stage));
+      need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0,
+                                    W, stage));
+  };
+  /* preference: staged rows with the eta table, staged rows without it, then
+   * (unless staging was forced) global rows, then fewer ants per workgroup */
+  recompute();
+  if (need > LDS_LIMIT) {
+    eta_in_lds = 0;
+    recompute();
+  }
+  if (need > LDS_LIMIT && want_stage < 0) {
+    stage = 0;
+    eta_in_lds = 1;
+    recompute();
+    if (need > LDS_LIMIT) {
+      eta_in_lds = 0;
+      recompute();
+    }
+  }
+  while (need > LDS_LIMIT && W > 1) {
+    --W;
+    recompute();
   }
   if (need > LDS_LIMIT)
     return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
@@ -384,6 +423,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
                    stage);
   c->W = W;
   c->stage = stage;
+  c->eta_lds = eta_in_lds;
   c->groups = (p->ants + W - 1) / W;
   c->lds_bytes = need;
 
@@ -395,11 +435,15 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if ((rc = dalloc(c, &c->d_hist, sizeof(HistEntry) * HIST * c->num_col))) return rc;
   if ((rc = dalloc(c, &c->d_stats, sizeof(unsigned long long) * ACO_STATS * c->num_col)))
     return rc;
+  if ((rc = dalloc(c, &c->d_done, sizeof(int) * c->num_col))) return rc;
   if ((rc = dalloc(c, &c->d_row_inst, sizeof(int) * (size_t)c->total_rows))) return rc;
   if ((rc = dalloc(c, &c->d_xy, sizeof(int2) * c->nodes_total))) return rc;
   if ((rc = dalloc(c, &c->d_dem, sizeof(uint16_t) * c->nodes_total))) return rc;
   if ((rc = dalloc(c, &c->d_dscore, sizeof(float) * c->nodes_total))) return rc;
   if ((rc = dalloc(c, &c->d_rec, sizeof(uint32_t) * c->rec_total))) return rc;
+  if ((rc = dalloc(c, &c->d_cidx, sizeof(uint16_t) * c->rec_total))) return rc;
+  if ((rc = dalloc(c, &c->d_slot, c->slot_total))) return rc;
+  if ((rc = dalloc(c, &c->d_score, sizeof(float) * c->score_total))) return rc;
   if ((rc = dalloc(c, &c->d_pher, c->pher_total))) return rc;
   if ((rc = dalloc(c, &c->d_tours, sizeof(int32_t) * c->tours_total))) return rc;
   if ((rc = dalloc(c, &c->d_len, sizeof(int32_t) * (size_t)ants))) return rc;
@@ -445,6 +489,9 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.dem = c->d_dem;
   P.dscore = c->d_dscore;
   P.rec = c->d_rec;
+  P.cidx = c->d_cidx;
+  P.slot_of = c->d_slot;
+  P.score = c->d_score;
   P.pher = c->d_pher;
   P.tours = c->d_tours;
   P.len = c->d_len;
@@ -456,6 +503,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.taupow = c->d_taupow;
   P.eta = c->d_eta;
   P.eta_len = c->eta_len;
+  P.eta_lds = c->eta_lds;
   P.q_evap = c->tables.q_evap;
   P.qdep = p->Q;
   P.fixed_epochs = 0;
@@ -464,11 +512,14 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.min_rel = 1e-3;
   P.hist = (void *)c->d_hist;
   P.stats = nullptr;
+  P.done = c->d_done;
 
-  HIPC(c, hipFuncSetAttribute((const void *)k_construct<true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
-  HIPC(c, hipFuncSetAttribute((const void *)k_construct<false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
+  const void *variants[] = {(const void *)k_construct<true, false>,
+                            (const void *)k_construct<false, false>,
+                            (const void *)k_construct<true, true>,
+                            (const void *)k_construct<false, true>};
+  for (const void *f : variants)
+    HIPC(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
   c->loaded = true;
   return ACO_HIP_OK;
 }
@@ -503,7 +554,9 @@ extern "C" int aco_hip_build_candidates(AcoHipContext *c) {
                        c->prm.knn_key, c->prm.depot_close_weight);
   }
   LAUNCHED(c, "k_knn");
-  return ACO_HIP_OK;
+  hipLaunchKernelGGL(k_build_slots, dim3(blocks), dim3(256), 0, c->stream, c->P, c->total_rows);
+  LAUNCHED(c, "k_build_slots");
+  return launch_refresh(c);
 }
 
 extern "C" int aco_hip_init_tau(AcoHipContext *c) {
@@ -515,6 +568,8 @@ extern "C" int aco_hip_init_tau(AcoHipContext *c) {
   hipLaunchKernelGGL(k_fill_tau, dim3(std::max(blocks, 1)), dim3(256), 0, c->stream, c->d_pher,
                      c->pher_total, word);
   LAUNCHED(c, "k_fill_tau");
+  int rc = launch_refresh(c);
+  if (rc) return rc;
   std::vector<AcoHipColonyState> st(c->num_col);
   for (AcoHipColonyState &s : st) {
     memset(&s, 0, sizeof(s));
@@ -532,42 +587,61 @@ static int launch_construct(AcoHipContext *c) {
   {
     Timed tm(c, T_CONSTRUCT);
     dim3 grid((unsigned)(c->num_col * c->groups)), block((unsigned)(64 * c->W));
-    if (c->stage)
-      hipLaunchKernelGGL(k_construct<true>, grid, block, c->lds_bytes, c->stream, c->P, c->groups,
-                         c->W);
+    const bool stats = c->P.stats != nullptr;
+    if (c->stage && !stats)
+      hipLaunchKernelGGL((k_construct<true, false>), grid, block, c->lds_bytes, c->stream, c->P,
+                         c->groups, c->W);
+    else if (!stats)
+      hipLaunchKernelGGL((k_construct<false, false>), grid, block, c->lds_bytes, c->stream, c->P,
+                         c->groups, c->W);
+    else if (c->stage)
+      hipLaunchKernelGGL((k_construct<true, true>), grid, block, c->lds_bytes, c->stream, c->P,
+                         c->groups, c->W);
     else
-      hipLaunchKernelGGL(k_construct<false>, grid, block, c->lds_bytes, c->stream, c->P,
+      hipLaunchKernelGGL((k_construct<false, true>), grid, block, c->lds_bytes, c->stream, c->P,
                          c->groups, c->W);
   }
   LAUNCHED(c, "k_construct");
   return ACO_HIP_OK;
 }
 
-static int launch_best(AcoHipContext *c) {
+static int launch_update(AcoHipContext *c, int mode) {
   {
-    Timed tm(c, T_BEST);
-    hipLaunchKernelGGL(k_select_best, dim3(c->num_col), dim3(256), 0, c->stream, c->P);
+    Timed tm(c, (mode & 1) ? T_BEST : T_DEPOSIT);
+    hipLaunchKernelGGL(k_update, dim3(c->num_col), dim3(1024), 0, c->stream, c->P, mode);
   }
-  LAUNCHED(c, "k_select_best");
+  LAUNCHED(c, "k_update");
   return ACO_HIP_OK;
 }
 
-static int launch_evap(AcoHipContext *c) {
-  const int x = std::max(1, std::min((c->max_pher_vec + 255) / 256, 2048));
+static int launch_evap(AcoHipContext *c, int from_ants) {
+  const int xm = std::max(1, std::min((c->max_mat_vec + 255) / 256, 1024));
+  const int xs = std::max(1, std::min((c->max_slots + 255) / 256, 512));
   {
     Timed tm(c, T_EVAP);
-    hipLaunchKernelGGL(k_evaporate, dim3(x, c->num_col), dim3(256), 0, c->stream, c->P);
+    hipLaunchKernelGGL(k_evaporate, dim3(xm + xs, c->num_col), dim3(256), 0, c->stream, c->P,
+                       xm, from_ants);
   }
   LAUNCHED(c, "k_evaporate");
   return ACO_HIP_OK;
 }
 
-static int launch_deposit(AcoHipContext *c, int which) {
+static int launch_evap_update(AcoHipContext *c) {
+  const int xm = std::max(1, std::min((c->max_mat_vec + 1023) / 1024, 512));
+  const int xs = std::max(1, std::min((c->max_slots + 1023) / 1024, 256));
   {
-    Timed tm(c, T_DEPOSIT);
-    hipLaunchKernelGGL(k_deposit, dim3(c->num_col), dim3(1024), 0, c->stream, c->P, which);
+    Timed tm(c, T_EVAP);
+    hipLaunchKernelGGL(k_evap_update, dim3(xm + xs, c->num_col), dim3(1024), 0, c->stream, c->P,
+                       xm);
   }
-  LAUNCHED(c, "k_deposit");
+  LAUNCHED(c, "k_evap_update");
+  return ACO_HIP_OK;
+}
+
+static int launch_refresh(AcoHipContext *c) {
+  const int xs = std::max(1, std::min((c->max_slots + 255) / 256, 512));
+  hipLaunchKernelGGL(k_refresh_scores, dim3(xs, c->num_col), dim3(256), 0, c->stream, c->P);
+  LAUNCHED(c, "k_refresh_scores");
   return ACO_HIP_OK;
 }
 
@@ -577,29 +651,28 @@ extern "C" int aco_hip_construct(AcoHipContext *c) {
 }
 extern "C" int aco_hip_select_best(AcoHipContext *c) {
   NEED_LOADED(c);
-  return launch_best(c);
+  return launch_update(c, 1);
 }
 extern "C" int aco_hip_evaporate(AcoHipContext *c) {
   NEED_LOADED(c);
-  return launch_evap(c);
+  return launch_evap(c, 0);
 }
 extern "C" int aco_hip_deposit_iter(AcoHipContext *c) {
   NEED_LOADED(c);
-  return launch_deposit(c, 0);
+  return launch_update(c, 2);
 }
 extern "C" int aco_hip_deposit_best(AcoHipContext *c) {
   NEED_LOADED(c);
-  return launch_deposit(c, 1);
+  return launch_update(c, 4);
 }
 
 static int launch_epochs(AcoHipContext *c, int count) {
   int rc;
   for (int e = 0; e < count; ++e) {
+    /* evaporation reads feasibility from the ants, so it runs before the
+     * fused best-selection + deposits workgroup: three launches per epoch */
     if ((rc = launch_construct(c))) return rc;
-    if ((rc = launch_best(c))) return rc;
-    if ((rc = launch_evap(c))) return rc;
-    if ((rc = launch_deposit(c, 0))) return rc;
-    if ((rc = launch_deposit(c, 1))) return rc;
+    if ((rc = launch_evap_update(c))) return rc;
   }
   return ACO_HIP_OK;
 }
@@ -816,6 +889,22 @@ extern "C" int aco_hip_set_tau(AcoHipContext *c, int colony, const uint8_t *tau)
     for (int t = 0; t < d.k; ++t) tc[i * d.k + t] = tau[i * n1 + (rec[i * d.k + t] & 0xFFFFu)];
   HIPC(c, hipMemcpy(c->d_pher + cd.pher_off, tau, n1 * n1, hipMemcpyHostToDevice));
   HIPC(c, hipMemcpy(c->d_pher + cd.tcand_off, tc.data(), cnt, hipMemcpyHostToDevice));
+  int rc2 = launch_refresh(c);
+  if (rc2) return rc2;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_scores(AcoHipContext *c, int colony, float *score) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_col(c, colony))) return rc;
+  if (!score) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null score");
+  const ColonyDesc &cd = c->col[colony];
+  const InstDesc &d = c->inst[cd.inst];
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(score, c->d_score + cd.score_off, sizeof(float) * (size_t)(d.n + 1) * d.k,
+                    hipMemcpyDeviceToHost));
   return ACO_HIP_OK;
 }
 

@@ -7,6 +7,11 @@
  *   dem     uint16 [same]                demand per node
  *   dscore  float  [same]                depot-close score 2*(1/d(i,0))^beta
  *   rec     uint32 [sum((n_i+1)*k)]      candidate records idx | nint(d) << 16
+ *   cidx    uint16 [same]                candidate indices (rec & 0xFFFF)
+ *   slot_of uint8  per instance (n+1)^2   1 + slot of v among u's candidates, 0 if
+ *                  v is not one (lets a deposit patch tau_cand in one load)
+ *   score   float  per colony (n+1)*k    taupow[tau_cand] * eta[d], refreshed by
+ *                  the evaporation kernel and patched by the deposits
  *   pher    uint8  per colony: (n+1)^2 pheromone matrix (row stride n+1,
  *                  the reference format), then the (n+1)*k candidate-aligned
  *                  copy tau_cand[i][t] == tau[i][cand[i][t]], 16-byte padded
@@ -29,7 +34,8 @@ struct InstDesc {
   int node_off;        /* into xy/dem/dscore (multiple of 4) */
   int row_base;        /* first kNN row of this instance */
   long long total_demand;
-  long long cand_off;  /* into rec (multiple of 4) */
+  long long cand_off;  /* into rec and cidx (multiple of 8) */
+  long long slot_off;  /* bytes into slot_of: (n+1)^2, slot+1 of v in row u or 0 */
   uint32_t seed;
   int pad;
 };
@@ -45,6 +51,8 @@ struct ColonyDesc {
   long long tour_off;  /* int32 elements into tours */
   int ant_off;         /* into len/cost/unvis */
   int best_off;        /* int32 elements into best */
+  long long score_off; /* float elements into score (multiple of 4) */
+  long long mat_bytes; /* (n+1)^2 rounded to 16: the matrix part of the block */
 };
 
 struct DevCtx {
@@ -56,6 +64,9 @@ struct DevCtx {
   const uint16_t *dem;
   float *dscore;
   uint32_t *rec;
+  uint16_t *cidx;      /* candidate index per slot (rec & 0xFFFF), per instance */
+  uint8_t *slot_of;    /* per instance (n+1)^2: 1 + slot of v in row u, 0 if none */
+  float *score;        /* taupow[tau_cand] * eta[d] per slot, per colony */
   uint8_t *pher;
   int32_t *tours;
   int32_t *len;
@@ -67,6 +78,7 @@ struct DevCtx {
   const float *taupow;
   const float *eta;
   int eta_len;
+  int eta_lds;         /* the eta table is staged in LDS (else read from global) */
   uint32_t q_evap;
   float qdep;
   int fixed_epochs;
@@ -75,6 +87,7 @@ struct DevCtx {
   double min_rel;
   void *hist;          /* HistEntry[colonies][ACO_HIST] per-epoch progress ring */
   unsigned long long *stats; /* [colonies][ACO_STATS] counters, NULL = off */
+  int *done;           /* per-colony block counter of k_evap_update (kept at 0) */
 };
 
 /* construct counters per colony: decisions, fallbacks, fallback cycles,
@@ -104,10 +117,13 @@ static inline __host__ __device__ int aco_significant(double prev, double now, d
 /* kernels (aco_kernels.hip) */
 __global__ void k_knn(DevCtx P, int total_rows, int knn_key, float depot_w);
 __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
-template <bool STAGE> __global__ void k_construct(DevCtx P, int groups_per_colony, int W);
-__global__ void k_select_best(DevCtx P);
-__global__ void k_evaporate(DevCtx P);
-__global__ void k_deposit(DevCtx P, int which);
+template <bool STAGE, bool STATS>
+__global__ void k_construct(DevCtx P, int groups_per_colony, int W);
+__global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
+__global__ void k_refresh_scores(DevCtx P);
+__global__ void k_update(DevCtx P, int mode);
+__global__ void k_evap_update(DevCtx P, int mat_blocks);
+__global__ void k_build_slots(DevCtx P, int total_rows);
 __global__ void k_deposit_tour(DevCtx P, int colony, const int32_t *tour, int len, float delta);
 __global__ void k_route_costs(DevCtx P, int instance, const int32_t *tours, const int32_t *lens,
                               int count, int stride, long long *out);

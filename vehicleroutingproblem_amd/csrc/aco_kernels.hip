@@ -129,6 +129,7 @@ __global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_k
         r = (uint32_t)j | (d << 16);
       }
       rec[t] = r;
+      P.cidx[I.cand_off + (size_t)i * k + t] = (uint16_t)(r & 0xFFFFu);
     }
     last = best;
   }
@@ -136,6 +137,27 @@ __global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_k
     const int2 p0 = xy[0];
     const int d0 = aco_nint_dist(pi.x, pi.y, p0.x, p0.y);
     P.dscore[I.node_off + i] = depot_w * P.eta[d0];
+  }
+}
+
+/* slot_of row i: zero, then 1 + t at column cidx[i][t] (first slot wins for a
+ * repeated index, which only the padding index 0 can be).  One wavefront per
+ * row. */
+__global__ void __launch_bounds__(256) k_build_slots(DevCtx P, int total_rows) {
+  const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= total_rows) return;
+  const InstDesc I = P.inst[P.row_inst[row]];
+  const int i = row - I.row_base;
+  const int n1 = I.n + 1, k = I.k;
+  uint8_t *so = P.slot_of + I.slot_off + (size_t)i * n1;
+  for (int j = lane; j < n1; j += 64) so[j] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane == 0) {
+    const uint16_t *ci = P.cidx + I.cand_off + (size_t)i * k;
+    for (int t = k - 1; t >= 0; --t) so[ci[t]] = (uint8_t)(t + 1);
   }
 }
 
@@ -152,13 +174,12 @@ __global__ void __launch_bounds__(256) k_fill_tau(uint8_t *pher, size_t bytes, u
 /* ---- tour construction --------------------------------------------------- */
 
 /* Copies `bytes` (multiple of 16, 16-byte aligned on both sides) with the
- * whole workgroup. */
+ * whole workgroup, eight loads in flight per thread. */
 __device__ __forceinline__ void wg_copy16(void *dst, const void *src, size_t bytes) {
   const int nvec = (int)(bytes / 16);
   const int nt = (int)blockDim.x;
   uint4 *d = reinterpret_cast<uint4 *>(dst);
   const uint4 *s = reinterpret_cast<const uint4 *>(src);
-  /* eight loads in flight per thread before the LDS stores */
   int e = (int)threadIdx.x;
   for (; e + 7 * nt < nvec; e += 8 * nt) {
     uint4 v[8];
@@ -176,8 +197,9 @@ struct FbOut {
   float T;
 };
 
-/* Weights of the 16 customers j0..j0+15 of one lane (0 when not eligible);
- * every load is issued before any is used. */
+/* Weights of the 16 customers j0..j0+15 of one lane (0 when not eligible).
+ * Every load is issued before any is used and the ineligible weights are
+ * zeroed by a multiply (weights are finite), so no lane branches. */
 __device__ __forceinline__ float fb_lane_weights(float (&w)[FB_V], int j0, int n, int room,
                                                  int2 pc, const int2 *xy, const float *taupow,
                                                  const float *eta, const uint16_t *state,
@@ -192,12 +214,18 @@ __device__ __forceinline__ float fb_lane_weights(float (&w)[FB_V], int j0, int n
     tq[i] = trow[jc];
     pj[i] = xy[jc];
   }
-  float s = 0.0f;
+  float tp[FB_V], et[FB_V];
 #pragma unroll
   for (int i = 0; i < FB_V; ++i) {
     const int d = aco_nint_dist(pc.x, pc.y, pj[i].x, pj[i].y);
-    const float v = taupow[tq[i]] * eta[d];
-    w[i] = (j0 + i <= n && (int)st[i] <= room) ? v : 0.0f;
+    tp[i] = taupow[tq[i]];
+    et[i] = eta[d];
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < FB_V; ++i) {
+    const float keep = (j0 + i <= n && (int)st[i] <= room) ? 1.0f : 0.0f;
+    w[i] = (tp[i] * et[i]) * keep;
     s = s + w[i];
   }
   return s;
@@ -262,9 +290,14 @@ __device__ __forceinline__ FbOut fallback_select(const int2 *xy, const float *ta
   return out;
 }
 
-template <bool STAGE>
+/* One wavefront per ant.  Per decision: lane t < k reads candidate t of the
+ * current node (index and precomputed score), then that node's state (its
+ * demand, or VISITED); a wavefront scan, one ballot and one bit-scan select
+ * the move.  All control flow is uniform (scalar). */
+template <bool STAGE, bool STATS>
 __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colony, int W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const unsigned long long t_kernel = STATS ? clock64() : 0ull;
   const int colony = blockIdx.x / groups_per_colony;
   const int group = blockIdx.x - colony * groups_per_colony;
   const ColonyDesc D = P.col[colony];
@@ -277,45 +310,49 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   const int wave = threadIdx.x >> 6;
 
   /* ---- LDS carve-up and staging ---- */
-  const unsigned long long t_kernel = P.stats ? clock64() : 0ull;
   size_t off = 0;
-  const uint32_t *c_rec;
-  const uint8_t *c_tc;
+  const uint16_t *c_idx;
+  const float *c_score;
   const float *c_dsc;
   const int2 *c_xy;
   if (STAGE) {
-    uint32_t *s_rec = reinterpret_cast<uint32_t *>(smem + off);
+    uint16_t *s_idx = reinterpret_cast<uint16_t *>(smem + off);
+    off += round16((size_t)n1 * k * 2);
+    float *s_score = reinterpret_cast<float *>(smem + off);
     off += round16((size_t)n1 * k * 4);
-    uint8_t *s_tc = smem + off;
-    off += round16((size_t)n1 * k);
     float *s_dsc = reinterpret_cast<float *>(smem + off);
     off += round16((size_t)n1 * 4);
     int2 *s_xy = reinterpret_cast<int2 *>(smem + off);
     off += round16((size_t)n1 * 8);
-    wg_copy16(s_rec, P.rec + I.cand_off, round16((size_t)n1 * k * 4));
-    wg_copy16(s_tc, P.pher + D.tcand_off, round16((size_t)n1 * k));
+    wg_copy16(s_idx, P.cidx + I.cand_off, round16((size_t)n1 * k * 2));
+    wg_copy16(s_score, P.score + D.score_off, round16((size_t)n1 * k * 4));
     wg_copy16(s_dsc, P.dscore + I.node_off, round16((size_t)n1 * 4));
     wg_copy16(s_xy, P.xy + I.node_off, round16((size_t)n1 * 8));
-    c_rec = s_rec;
-    c_tc = s_tc;
+    c_idx = s_idx;
+    c_score = s_score;
     c_dsc = s_dsc;
     c_xy = s_xy;
   } else {
-    c_rec = P.rec + I.cand_off;
-    c_tc = P.pher + D.tcand_off;
+    c_idx = P.cidx + I.cand_off;
+    c_score = P.score + D.score_off;
     c_dsc = P.dscore + I.node_off;
     c_xy = P.xy + I.node_off;
   }
   float *s_taupow = reinterpret_cast<float *>(smem + off);
   off += 256 * 4;
   float *s_eta = reinterpret_cast<float *>(smem + off);
-  off += round16((size_t)P.eta_len * 4);
   wg_copy16(s_taupow, P.taupow, 256 * 4);
-  wg_copy16(s_eta, P.eta, round16((size_t)P.eta_len * 4));
-  const size_t state_stride = round16((size_t)n1 * 2);
+  if (P.eta_lds) {
+    off += round16((size_t)P.eta_len * 4);
+    wg_copy16(s_eta, P.eta, round16((size_t)P.eta_len * 4));
+  }
+  /* large coordinate ranges keep the eta table in global memory (only the
+   * fallback reads it) */
+  const float *c_eta = P.eta_lds ? (const float *)s_eta : P.eta;
+  const int state_stride = (int)round16((size_t)n1 * 2);
   uint16_t *s_state = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * state_stride);
   __syncthreads();
-  if (P.stats && threadIdx.x == 0)
+  if (STATS && threadIdx.x == 0)
     atomicAdd(P.stats + (size_t)colony * ACO_STATS + 5, clock64() - t_kernel);
 
   const int ant = group * W + wave;
@@ -336,13 +373,14 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   const uint32_t cid = (uint32_t)D.colony_in_inst;
   const int rows = (k + 15) >> 4;
   const int klane = lane < k ? lane : 0;
+  const bool in_k = lane < k;
 
-  const unsigned long long t_start = P.stats ? clock64() : 0ull;
+  const unsigned long long t_start = STATS ? clock64() : 0ull;
   unsigned long long fallbacks = 0, fb_cycles = 0;
   int rem_cnt = n;
-  long long rem_dem = I.total_demand;
+  int rem_dem = (int)I.total_demand;
   uint32_t s = 0;
-  uint32_t rng0 = 0, rng1 = 0;
+  float rf0 = 0.0f, rf1 = 0.0f; /* lane l: uniform draws of decision (s & ~63) + l */
   int pos = 0;
   int tb = 0;      /* tour buffer: lane (p & 63) holds entry p of the current block */
   int carry = 0;   /* last entry of the previous block */
@@ -352,7 +390,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
     const int e = tb;
     int prev = __shfl_up(e, 1, 64);
     if (lane == 0) prev = carry;
-    long long dd = 0;
+    int dd = 0;
     if (lane < cnt) {
       if (base + lane >= 1) {
         const int2 a = c_xy[prev], b = c_xy[e];
@@ -363,71 +401,64 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
     cost += wave_sum_i64(dd);
     carry = __builtin_amdgcn_readlane(e, cnt - 1);
   };
-  auto append = [&](int node) {
-    ++pos;
-    const int slot = pos & 63;
-    if (lane == slot) tb = node;
-    if (slot == 63) flush(pos - 63, 64);
-  };
 
   for (int veh = 0; veh < K; ++veh) {
     int cur = 0;
     int load = 0;
+    const int close_cap = (K - veh - 1) * Q; /* close allowed iff rem_dem <= close_cap */
     for (;;) {
       if (rem_cnt <= 0 || load >= Q) break; /* aco_cuda_kernels.cu:492 */
       const int room = Q - load;
-      if ((s & 63u) == 0u) {
+      const int sl = (int)(s & 63u);
+      if (sl == 0) {
         uint32_t r[4];
         aco_philox4x32_10(s + (uint32_t)lane, (uint32_t)ant, (uint32_t)epoch, cid, key0,
                           ACO_PHILOX_KEY1, r);
-        rng0 = r[0];
-        rng1 = r[1];
+        rf0 = aco_u01(r[0]);
+        rf1 = aco_u01(r[1]);
       }
-      const uint32_t u0 = (uint32_t)__builtin_amdgcn_readlane((int)rng0, (int)(s & 63u));
-      const uint32_t u1 = (uint32_t)__builtin_amdgcn_readlane((int)rng1, (int)(s & 63u));
       ++s;
 
-      /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322):
-       * lanes >= k read slot 0 and are masked out */
-      const size_t e = (size_t)cur * k + klane;
-      const uint32_t rc = c_rec[e];
-      const uint32_t tq = c_tc[e];
-      const int j = lane < k ? (int)(rc & 0xFFFFu) : 0;
-      const uint32_t st = s_state[j];
-      const float v = s_taupow[tq] * s_eta[rc >> 16];
-      const float w = ((int)st <= room) ? v : 0.0f;
+      /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322) */
+      const int e = cur * k + klane;
+      const int j = in_k ? (int)c_idx[e] : 0;
+      const float sc = c_score[e];
+      const float dsc_cur = c_dsc[cur]; /* issued with the first batch of loads */
+      const int st = (int)s_state[j];
+      const float w = (st <= room) ? sc : 0.0f;
       const float Pf = wave_scan_rows(w, lane, rows);
       float T = f_readlane(Pf, k - 1);
-      int move, mdem = 0;
+      int move, mdem;
       if (T > 0.0f) {
-        const float thr = aco_u01(u0) * T;
+        const float thr = f_readlane(rf0, sl) * T;
         const uint64_t m = ballot64(w > 0.0f && Pf > thr);
         const int sel = m ? first_lane(m) : last_lane(ballot64(w > 0.0f));
         move = __builtin_amdgcn_readlane(j, sel);
-        mdem = __builtin_amdgcn_readlane((int)st, sel);
+        mdem = __builtin_amdgcn_readlane(st, sel);
       } else {
-        const unsigned long long f0 = P.stats ? clock64() : 0ull;
-        const FbOut fb = fallback_select(c_xy, s_taupow, s_eta, s_state, tau + (size_t)cur * n1,
-                                         n, cur, room, aco_u01(u0), lane);
+        const unsigned long long f0 = STATS ? clock64() : 0ull;
+        const FbOut fb = fallback_select(c_xy, s_taupow, c_eta, s_state, tau + (size_t)cur * n1,
+                                         n, cur, room, f_readlane(rf0, sl), lane);
         move = fb.move;
         mdem = fb.mdem;
         T = fb.T;
-        if (P.stats) {
+        if (STATS) {
           fb_cycles += clock64() - f0;
           ++fallbacks;
         }
       }
       /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand */
-      const bool close_allowed = rem_dem <= (long long)(K - veh - 1) * Q;
-      const float dsc =
-          close_allowed ? __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(c_dsc[cur])))
-                        : 0.0f;
-      if (move > 0 && dsc > 0.0f) {
-        const float thr2 = aco_u01(u1) * (T + dsc);
-        if (thr2 >= T) move = 0;
+      if (move > 0 && rem_dem <= close_cap) {
+        const float dsc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dsc_cur)));
+        if (dsc > 0.0f) {
+          const float thr2 = f_readlane(rf1, sl) * (T + dsc);
+          if (thr2 >= T) move = 0;
+        }
       }
       if (move <= 0) break;
-      append(move);
+      ++pos;
+      if (lane == (pos & 63)) tb = move;
+      if ((pos & 63) == 63) flush(pos - 63, 64);
       if (lane == 0) s_state[move] = (uint16_t)VISITED;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -437,7 +468,10 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
       --rem_cnt;
       cur = move;
     }
-    append(0); /* route close (aco_cuda_kernels.cu:547-552) */
+    /* route close (aco_cuda_kernels.cu:547-552) */
+    ++pos;
+    if (lane == (pos & 63)) tb = 0;
+    if ((pos & 63) == 63) flush(pos - 63, 64);
   }
   if ((pos & 63) != 63) flush(pos & ~63, (pos & 63) + 1);
   if (lane == 0) {
@@ -445,7 +479,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
     P.len[a] = pos + 1;
     P.cost[a] = cost;
     P.unvis[a] = rem_cnt;
-    if (P.stats) {
+    if (STATS) {
       unsigned long long *sc = P.stats + (size_t)colony * ACO_STATS;
       atomicAdd(sc + 0, (unsigned long long)s);
       atomicAdd(sc + 1, fallbacks);
@@ -456,26 +490,110 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   }
 }
 
-template __global__ void k_construct<true>(DevCtx, int, int);
-template __global__ void k_construct<false>(DevCtx, int, int);
+template __global__ void k_construct<true, false>(DevCtx, int, int);
+template __global__ void k_construct<false, false>(DevCtx, int, int);
+template __global__ void k_construct<true, true>(DevCtx, int, int);
+template __global__ void k_construct<false, true>(DevCtx, int, int);
 
-/* ---- best selection and stop rules --------------------------------------- */
+/* ---- evaporation ---------------------------------------------------------- */
+
+__device__ __forceinline__ uint32_t evap_word(uint32_t x, uint32_t d) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    uint32_t q = (x >> (8 * b)) & 0xFFu;
+    q = (q > d) ? q - d : 0u; /* q > qmin + delta ? q - delta : qmin, qmin = 0 */
+    r |= q << (8 * b);
+  }
+  return r;
+}
+
+/* Saturating subtract (kernel_evaporate_tau, aco_cuda_kernels.cu:145-155).
+ * Blocks [0, mat_blocks) stream the (n+1)^2 matrix 16 bytes per lane; the
+ * remaining blocks evaporate the candidate-aligned copy slot by slot and
+ * refresh each slot's score taupow[q] * eta[d] (the per-epoch score refresh
+ * of the reference OpenMP path, aco_parallel.c:535-540).
+ * from_ants = 0: evaporate iff the last best selection found a feasible ant
+ * (the reference order, aco_cuda.cu:355-385).  from_ants = 1 (fused epoch,
+ * before the best selection): iff the colony is not halted and some ant of
+ * this epoch is feasible - the same condition, read from the ants. */
+__global__ void __launch_bounds__(256) k_evaporate(DevCtx P, int mat_blocks, int from_ants) {
+  const int colony = blockIdx.y;
+  const AcoHipColonyState *S = P.state + colony;
+  const ColonyDesc D = P.col[colony];
+  if (from_ants) {
+    if (S->halted) return;
+    int any = 0;
+    for (int a = threadIdx.x; a < D.m; a += blockDim.x) any |= P.unvis[D.ant_off + a] == 0;
+    if (!__syncthreads_or(any)) return;
+  } else if (!S->found) {
+    return;
+  }
+  const uint32_t d = P.q_evap;
+  if ((int)blockIdx.x < mat_blocks) {
+    uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
+    const size_t nvec = (size_t)D.mat_bytes / 16;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nvec;
+         e += (size_t)mat_blocks * blockDim.x) {
+      uint4 v = base[e];
+      v.x = evap_word(v.x, d);
+      v.y = evap_word(v.y, d);
+      v.z = evap_word(v.z, d);
+      v.w = evap_word(v.w, d);
+      base[e] = v;
+    }
+    return;
+  }
+  const InstDesc I = P.inst[D.inst];
+  const size_t slots = (size_t)(I.n + 1) * I.k;
+  uint8_t *tc = P.pher + D.tcand_off;
+  float *score = P.score + D.score_off;
+  const uint32_t *rec = P.rec + I.cand_off;
+  const size_t nb = gridDim.x - mat_blocks;
+  for (size_t e = (blockIdx.x - mat_blocks) * (size_t)blockDim.x + threadIdx.x; e < slots;
+       e += nb * blockDim.x) {
+    uint32_t q = tc[e];
+    q = q > d ? q - d : 0u;
+    tc[e] = (uint8_t)q;
+    score[e] = P.taupow[q] * P.eta[rec[e] >> 16];
+  }
+}
+
+/* score[slot] = taupow[tau_cand[slot]] * eta[d(slot)] for every slot (after
+ * pheromone init, candidate build or a pheromone upload). */
+__global__ void __launch_bounds__(256) k_refresh_scores(DevCtx P) {
+  const int colony = blockIdx.y;
+  const ColonyDesc D = P.col[colony];
+  const InstDesc I = P.inst[D.inst];
+  const size_t slots = (size_t)(I.n + 1) * I.k;
+  const uint8_t *tc = P.pher + D.tcand_off;
+  float *score = P.score + D.score_off;
+  const uint32_t *rec = P.rec + I.cand_off;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < slots;
+       e += (size_t)gridDim.x * blockDim.x)
+    score[e] = P.taupow[tc[e]] * P.eta[rec[e] >> 16];
+}
+
+/* ---- best selection, stop rules and deposits (one workgroup per colony) --- */
+
+/* Tables of the update workgroup, staged in LDS. */
+struct UpdTables {
+  const float *deq;    /* 256 */
+  const float *thr;    /* 255 */
+  const float *taupow; /* 256 */
+  const float *eta;    /* eta table (global memory, L2-resident) */
+};
 
 /* Iteration best over feasible ants (lowest cost, ties to the lower ant) and
- * the global-best / stagnation bookkeeping of aco_cuda.cu:104-126,331-399,
- * all on device. */
-__global__ void __launch_bounds__(256) k_select_best(DevCtx P) {
-  __shared__ unsigned long long s_min[4];
-  __shared__ int s_cnt[4];
-  __shared__ int s_copy;
-  const int colony = blockIdx.x;
+ * the global-best / stagnation bookkeeping of aco_cuda.cu:104-126,331-399.
+ * Returns (to every thread) whether a feasible ant was found. */
+__device__ bool select_best_block(const DevCtx &P, int colony) {
+  __shared__ unsigned long long s_min[16];
+  __shared__ int s_cnt[16];
+  __shared__ int s_copy, s_found;
   AcoHipColonyState *S = P.state + colony;
   const ColonyDesc D = P.col[colony];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (S->halted) {
-    if (threadIdx.x == 0) S->found = 0;
-    return;
-  }
   unsigned long long key = ~0ull;
   int cnt = 0;
   for (int a = threadIdx.x; a < D.m; a += blockDim.x) {
@@ -540,6 +658,7 @@ __global__ void __launch_bounds__(256) k_select_best(DevCtx P) {
     h->improved = S->improved;
     h->halted = S->halted;
     h->global_best = S->global_best_cost;
+    s_found = S->found;
   }
   __syncthreads();
   const int ant = s_copy;
@@ -548,52 +667,39 @@ __global__ void __launch_bounds__(256) k_select_best(DevCtx P) {
     int32_t *dst = P.best + D.best_off;
     for (int p = threadIdx.x; p < D.stride; p += blockDim.x) dst[p] = src[p];
   }
+  __syncthreads();
+  return s_found != 0;
 }
 
-/* ---- evaporation ---------------------------------------------------------- */
+__device__ __forceinline__ uint32_t dep_new(const UpdTables &T, uint32_t q, float delta) {
+  return aco_quantize_thr(T.thr, T.deq[q] + delta);
+}
 
-__device__ __forceinline__ uint32_t evap_word(uint32_t x, uint32_t d) {
-  uint32_t r = 0;
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    uint32_t q = (x >> (8 * b)) & 0xFFu;
-    q = (q > d) ? q - d : 0u; /* q > qmin + delta ? q - delta : qmin, qmin = 0 */
-    r |= q << (8 * b);
+/* Patch tau_cand and score of row u for candidate v after tau[u][v] became
+ * `val`.  slot_of gives the slot in the same memory round trip as the
+ * pheromone bytes; d(u,v) comes from the coordinates (the kNN stored the same
+ * nint distance).  Rows with padding slots (k >= n, index 0 repeated) patch
+ * every matching slot by scanning the row. */
+__device__ __forceinline__ void patch_slot(const DevCtx &P, const UpdTables &T,
+                                           const InstDesc &I, uint8_t *__restrict__ tc,
+                                           float *__restrict__ score, int u, int v, int slot1,
+                                           int d, uint8_t val) {
+  const int k = I.k;
+  if (I.k >= I.n) {
+    const uint32_t *__restrict__ row = P.rec + I.cand_off + (size_t)u * k;
+    for (int t = 0; t < k; ++t) {
+      const uint32_t r = row[t];
+      if ((int)(r & 0xFFFFu) == v) {
+        tc[(size_t)u * k + t] = val;
+        score[(size_t)u * k + t] = T.taupow[val] * T.eta[r >> 16];
+      }
+    }
+    return;
   }
-  return r;
-}
-
-/* Saturating subtract over the colony's whole pheromone block (matrix and
- * candidate copy): kernel_evaporate_tau (aco_cuda_kernels.cu:145-155). */
-__global__ void __launch_bounds__(256) k_evaporate(DevCtx P) {
-  const int colony = blockIdx.y;
-  if (!P.state[colony].found) return;
-  const ColonyDesc D = P.col[colony];
-  uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
-  const size_t nvec = (size_t)D.pher_bytes / 16;
-  const uint32_t d = P.q_evap;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nvec;
-       e += (size_t)gridDim.x * blockDim.x) {
-    uint4 v = base[e];
-    v.x = evap_word(v.x, d);
-    v.y = evap_word(v.y, d);
-    v.z = evap_word(v.z, d);
-    v.w = evap_word(v.w, d);
-    base[e] = v;
+  if (slot1) {
+    tc[(size_t)u * k + slot1 - 1] = val;
+    score[(size_t)u * k + slot1 - 1] = T.taupow[val] * T.eta[d];
   }
-}
-
-/* ---- deposits -------------------------------------------------------------- */
-
-__device__ __forceinline__ uint32_t dep_f(const DevCtx &P, uint32_t q, float delta) {
-  const uint32_t nq = aco_quantize_thr(P.thr, P.deq[q] + delta);
-  return nq > q ? nq : q;
-}
-
-__device__ __forceinline__ void patch_cand(const uint32_t *rec, uint8_t *tc, int k, int u, int v,
-                                           uint8_t val) {
-  for (int t = 0; t < k; ++t)
-    if ((int)(rec[(size_t)u * k + t] & 0xFFFFu) == v) tc[(size_t)u * k + t] = val;
 }
 
 /* Deposit one tour: per edge new = quantize(dequantize(tau[u][v]) + delta),
@@ -602,14 +708,18 @@ __device__ __forceinline__ void patch_cand(const uint32_t *rec, uint8_t *tc, int
  * edge; the only entries two edges share are those of a single-customer
  * route 0-c-0 (one thread does both edges) and the (0,0) entry of empty
  * routes (one thread applies them all), so the result equals the sequential
- * edge order.  The candidate-aligned copy is patched alongside. */
-__device__ void deposit_tour(const DevCtx &P, const ColonyDesc &D, const InstDesc &I,
-                             const int32_t *tour, int len, float delta) {
+ * edge order.  The candidate-aligned copy and its scores are patched
+ * alongside.  Two dependent memory round trips per edge.  Ends with a
+ * workgroup barrier. */
+__device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDesc &D,
+                             const InstDesc &I, const int32_t *tour, int len, float delta) {
   __shared__ int s_zz_count, s_zz_first;
-  const int n1 = I.n + 1, k = I.k;
-  uint8_t *tau = P.pher + D.pher_off;
-  uint8_t *tc = P.pher + D.tcand_off;
-  const uint32_t *rec = P.rec + I.cand_off;
+  const int n1 = I.n + 1;
+  uint8_t *__restrict__ tau = P.pher + D.pher_off;
+  uint8_t *__restrict__ tc = P.pher + D.tcand_off;
+  float *__restrict__ score = P.score + D.score_off;
+  const uint8_t *__restrict__ so = P.slot_of + I.slot_off;
+  const int2 *__restrict__ xy = P.xy + I.node_off;
   if (threadIdx.x == 0) {
     s_zz_count = 0;
     s_zz_first = 0x7FFFFFFF;
@@ -624,56 +734,160 @@ __device__ void deposit_tour(const DevCtx &P, const ColonyDesc &D, const InstDes
   __syncthreads();
   for (int p = threadIdx.x; p + 1 < len; p += blockDim.x) {
     const int u = tour[p], v = tour[p + 1];
+    const int um1 = p >= 1 ? tour[p - 1] : -1;
+    const int vp1 = p + 2 < len ? tour[p + 2] : -1;
     if (u == 0 && v == 0) {
       if (p == s_zz_first) {
         uint32_t x = tau[0];
-        for (int r = 0; r < s_zz_count; ++r) x = dep_f(P, x, delta);
+        for (int r = 0; r < s_zz_count; ++r) {
+          const uint32_t nq = dep_new(T, x, delta);
+          x = nq > x ? nq : x;
+        }
         tau[0] = (uint8_t)x;
-        patch_cand(rec, tc, k, 0, 0, (uint8_t)x);
+        patch_slot(P, T, I, tc, score, 0, 0, so[0], 0, (uint8_t)x);
       }
       continue;
     }
-    if (u != 0 && v == 0 && p >= 1 && tour[p - 1] == 0) continue; /* done by thread p-1 */
-    if (u == 0 && p + 2 < len && tour[p + 2] == 0) {
-      /* single-customer route 0 -> v -> 0: both edges in order */
-      uint32_t t0v = tau[v], tv0 = tau[(size_t)v * n1];
-      uint32_t nq = aco_quantize_thr(P.thr, P.deq[t0v] + delta);
-      t0v = nq > t0v ? nq : t0v;
-      tv0 = nq > tv0 ? nq : tv0;
-      nq = aco_quantize_thr(P.thr, P.deq[tv0] + delta);
-      tv0 = nq > tv0 ? nq : tv0;
-      t0v = nq > t0v ? nq : t0v;
-      tau[v] = (uint8_t)t0v;
-      tau[(size_t)v * n1] = (uint8_t)tv0;
-      patch_cand(rec, tc, k, 0, v, (uint8_t)t0v);
-      patch_cand(rec, tc, k, v, 0, (uint8_t)tv0);
-      continue;
-    }
+    if (u != 0 && v == 0 && um1 == 0) continue; /* done by thread p-1 */
     const uint32_t a = tau[(size_t)u * n1 + v];
-    const uint32_t nq = aco_quantize_thr(P.thr, P.deq[a] + delta);
-    const uint32_t tuv = nq > a ? nq : a;
     const uint32_t b = tau[(size_t)v * n1 + u];
-    const uint32_t tvu = nq > b ? nq : b;
+    const int su = so[(size_t)u * n1 + v];
+    const int sv = so[(size_t)v * n1 + u];
+    const int2 pu = xy[u], pv = xy[v];
+    const int d = aco_nint_dist(pu.x, pu.y, pv.x, pv.y);
+    uint32_t tuv, tvu;
+    if (u == 0 && vp1 == 0) {
+      /* single-customer route 0 -> v -> 0: both edges in order */
+      uint32_t nq = dep_new(T, a, delta);
+      tuv = nq > a ? nq : a;
+      tvu = nq > b ? nq : b;
+      nq = dep_new(T, tvu, delta);
+      tvu = nq > tvu ? nq : tvu;
+      tuv = nq > tuv ? nq : tuv;
+    } else {
+      const uint32_t nq = dep_new(T, a, delta);
+      tuv = nq > a ? nq : a;
+      tvu = nq > b ? nq : b;
+    }
     tau[(size_t)u * n1 + v] = (uint8_t)tuv;
     tau[(size_t)v * n1 + u] = (uint8_t)tvu;
-    patch_cand(rec, tc, k, u, v, (uint8_t)tuv);
-    patch_cand(rec, tc, k, v, u, (uint8_t)tvu);
+    patch_slot(P, T, I, tc, score, u, v, su, d, (uint8_t)tuv);
+    patch_slot(P, T, I, tc, score, v, u, sv, d, (uint8_t)tvu);
   }
+  __syncthreads();
 }
 
-__global__ void __launch_bounds__(1024) k_deposit(DevCtx P, int which) {
+__device__ __forceinline__ UpdTables stage_tables(const DevCtx &P) {
+  __shared__ float s_deq[256], s_thr[256], s_tp[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    s_deq[i] = P.deq[i];
+    s_thr[i] = i < 255 ? P.thr[i] : 0.0f;
+    s_tp[i] = P.taupow[i];
+  }
+  __syncthreads();
+  UpdTables T;
+  T.deq = s_deq;
+  T.thr = s_thr;
+  T.taupow = s_tp;
+  T.eta = P.eta;
+  return T;
+}
+
+/* mode bit 1: best selection and stop rules; bit 2: iteration-best deposit;
+ * bit 4: global-best deposit.  Deposits run only when the last best
+ * selection found a feasible ant (aco_cuda.cu:355-396).  Global writes of one
+ * phase are visible to the next through the workgroup barriers. */
+__global__ void __launch_bounds__(1024) k_update(DevCtx P, int mode) {
   const int colony = blockIdx.x;
-  const AcoHipColonyState *S = P.state + colony;
-  if (!S->found) return;
+  AcoHipColonyState *S = P.state + colony;
+  if (mode & 1) {
+    if (S->halted) {
+      if (threadIdx.x == 0) S->found = 0;
+      return;
+    }
+    if (!select_best_block(P, colony)) return;
+  } else if (!S->found) {
+    return;
+  }
+  if (!(mode & 6)) return;
   const ColonyDesc D = P.col[colony];
   const InstDesc I = P.inst[D.inst];
-  if (which == 0) {
+  const UpdTables T = stage_tables(P);
+  if (mode & 2) {
     const int ant = S->best_ant;
-    deposit_tour(P, D, I, P.tours + D.tour_off + (size_t)ant * D.stride, P.len[D.ant_off + ant],
-                 S->dep_iter);
-  } else {
-    deposit_tour(P, D, I, P.best + D.best_off, S->best_len, S->dep_best);
+    deposit_tour(P, T, D, I, P.tours + D.tour_off + (size_t)ant * D.stride,
+                 P.len[D.ant_off + ant], S->dep_iter);
   }
+  if (mode & 4) deposit_tour(P, T, D, I, P.best + D.best_off, S->best_len, S->dep_best);
+}
+
+/* One epoch's pheromone update in one launch: every block evaporates its
+ * share (deciding feasibility from the ants, as k_evaporate with from_ants),
+ * then the last block to finish - found with a device-wide counter, no
+ * waiting - runs the best selection and both deposits (k_update mode 7).
+ * __threadfence() before the counter increment and after reading it makes
+ * every block's evaporation visible to that last block. */
+__global__ void __launch_bounds__(1024) k_evap_update(DevCtx P, int mat_blocks) {
+  __shared__ int s_last;
+  const int colony = blockIdx.y;
+  AcoHipColonyState *S = P.state + colony;
+  const ColonyDesc D = P.col[colony];
+  const bool halted = S->halted != 0;
+  int any = 0;
+  if (!halted)
+    for (int a = threadIdx.x; a < D.m; a += blockDim.x) any |= P.unvis[D.ant_off + a] == 0;
+  const bool run = !halted && __syncthreads_or(any);
+  if (run) {
+    const uint32_t d = P.q_evap;
+    if ((int)blockIdx.x < mat_blocks) {
+      uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
+      const size_t nvec = (size_t)D.mat_bytes / 16;
+      for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nvec;
+           e += (size_t)mat_blocks * blockDim.x) {
+        uint4 v = base[e];
+        v.x = evap_word(v.x, d);
+        v.y = evap_word(v.y, d);
+        v.z = evap_word(v.z, d);
+        v.w = evap_word(v.w, d);
+        base[e] = v;
+      }
+    } else {
+      const InstDesc I = P.inst[D.inst];
+      const size_t slots = (size_t)(I.n + 1) * I.k;
+      uint8_t *tc = P.pher + D.tcand_off;
+      float *score = P.score + D.score_off;
+      const uint32_t *rec = P.rec + I.cand_off;
+      const size_t nb = gridDim.x - mat_blocks;
+      for (size_t e = (blockIdx.x - mat_blocks) * (size_t)blockDim.x + threadIdx.x; e < slots;
+           e += nb * blockDim.x) {
+        uint32_t q = tc[e];
+        q = q > d ? q - d : 0u;
+        tc[e] = (uint8_t)q;
+        score[e] = P.taupow[q] * P.eta[rec[e] >> 16];
+      }
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int prev = atomicAdd(P.done + colony, 1);
+    s_last = prev == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (threadIdx.x == 0) P.done[colony] = 0; /* ready for the next epoch */
+  if (halted) {
+    if (threadIdx.x == 0) S->found = 0;
+    return;
+  }
+  if (!select_best_block(P, colony)) return;
+  const InstDesc I = P.inst[D.inst];
+  const UpdTables T = stage_tables(P);
+  const int ant = S->best_ant;
+  deposit_tour(P, T, D, I, P.tours + D.tour_off + (size_t)ant * D.stride, P.len[D.ant_off + ant],
+               S->dep_iter);
+  deposit_tour(P, T, D, I, P.best + D.best_off, S->best_len, S->dep_best);
 }
 
 /* Deposit of a host-provided tour (staged in the scratch tour buffer). */
@@ -681,7 +895,8 @@ __global__ void __launch_bounds__(1024) k_deposit_tour(DevCtx P, int colony, con
                                                         int len, float delta) {
   const ColonyDesc D = P.col[colony];
   const InstDesc I = P.inst[D.inst];
-  deposit_tour(P, D, I, tour, len, delta);
+  const UpdTables T = stage_tables(P);
+  deposit_tour(P, T, D, I, tour, len, delta);
 }
 
 /* ---- route-cost reduction ------------------------------------------------- */

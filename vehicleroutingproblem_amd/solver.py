@@ -274,6 +274,13 @@ class HipSolver:
                                              N.ptr(tc, C.c_uint8)))
         return tau, tc
 
+    def scores(self, colony: int = 0) -> np.ndarray:
+        inst = self.instance_of(colony)
+        k = self.cand_k(colony // self.params.colonies)
+        out = np.zeros((inst.n + 1, k), np.float32)
+        self._check(self.lib.aco_hip_get_scores(self._ctx, colony, N.ptr(out, C.c_float)))
+        return out
+
     def set_tau(self, tau: np.ndarray, colony: int = 0):
         tau = np.ascontiguousarray(tau, dtype=np.uint8)
         self._check(self.lib.aco_hip_set_tau(self._ctx, colony, N.ptr(tau, C.c_uint8)))
