@@ -180,6 +180,9 @@ def main():
             "fallback_rate": round(dev["fallbacks"] / max(1, dev["decisions"]), 4),
             "fallback_cycle_share": round(dev["fallback_cycles"] / max(1, dev["ant_cycles"]), 4),
             "cycles_per_decision": round(dev["ant_cycles"] / max(1, dev["decisions"]), 1)},
+        "update_phases_us": {
+            k: round(1e6 * dev[k + "_ticks"] / max(1, dev["wall_clock_hz"]) / max(1, dev["updates"]),
+                     2) for k in ("evap", "select", "deposit")},
     }
     s.close()
     if not args.no_cpu:

@@ -90,6 +90,9 @@ typedef struct {
   int knn_key;           /* ACO_HIP_KNN_* */
   int ants_per_group;    /* ants (wavefronts) per workgroup, 0 = auto */
   int stage_lds;         /* -1 auto, 0 read candidate rows from global memory, 1 stage in LDS */
+  int fallback_matrix;   /* -1 auto, 0 off, 1 on: keep the all-pairs fallback weights
+                            taupow[tau[u][v]]*eta[d(u,v)] as a float matrix per colony
+                            ((n+1) x roundup(n+1,16) floats) instead of recomputing them */
 } AcoHipParams;
 
 /* Device-side state of one colony, read back with aco_hip_get_colony(). */
@@ -195,6 +198,11 @@ int aco_hip_set_tau(AcoHipContext *ctx, int colony, const uint8_t *tau);
 /* (n+1)*k candidate scores taupow[tau_cand] * eta^beta, the weights the
  * construction samples (the reference OpenMP score matrix, aco_parallel.c:535-540). */
 int aco_hip_get_scores(AcoHipContext *ctx, int colony, float *score);
+/* (n+1)*(n+1) fallback weights taupow[tau[u][v]] * eta^beta(d(u,v)) of the
+ * all-pairs matrix (the weights select_global_fallback computes per step,
+ * aco_cuda_kernels.cu:336-430).  ACO_HIP_ERR_INVALID_INPUT when the matrix is
+ * off (AcoHipParams.fallback_matrix). */
+int aco_hip_get_fallback_weights(AcoHipContext *ctx, int colony, float *weights);
 /* tours: ants*tour_stride int32; len, unvisited: ants int32; cost: ants int64. */
 int aco_hip_get_tours(AcoHipContext *ctx, int colony, int32_t *tours, int32_t *len,
                       int64_t *cost, int32_t *unvisited);
@@ -223,6 +231,12 @@ typedef struct {
   uint64_t ant_cycles;       /* shader clocks from ant start to tour end */
   uint64_t ants;             /* tours counted */
   uint64_t staging_cycles;   /* shader clocks per workgroup spent staging LDS, summed */
+  /* pheromone-update kernel, in constant-rate wall-clock ticks (wall_clock_hz) */
+  uint64_t updates;          /* epochs whose update ran the best selection */
+  uint64_t evap_ticks;       /* first block start -> last block done evaporating */
+  uint64_t select_ticks;     /* best selection and stop rules */
+  uint64_t deposit_ticks;    /* iteration-best + global-best deposits */
+  uint64_t wall_clock_hz;    /* tick rate of the three fields above */
 } AcoHipStats;
 
 /* Enable (1) / disable (0) the counters; enabling clears them. */

@@ -328,13 +328,14 @@ void oracle_set_stop_rules(OracleColony *o, int stagnation_epochs, double min_re
 
 /* Fallback roulette over every unvisited customer that fits, weight
  * tau^alpha * eta^beta (select_global_fallback, aco_cuda_kernels.cu:336-430,
- * restated as an exact two-pass roulette).  Customer j = 1 + c*1024 + l*16 + i
- * belongs to chunk c, lane l, slot i.  Returns the node or -1 and the total. */
+ * restated as an exact two-pass roulette).  Node j = c*1024 + l*16 + i
+ * (0 <= j <= n) belongs to chunk c, lane l, slot i; the depot is visited from
+ * the start and never carries weight.  Returns the node or -1 and the total. */
 static int fallback_select(const OracleColony *o, const uint16_t *state, int cur, int room,
                            float u, float *total_out) {
   const int n = o->n;
   const uint8_t *trow = o->tau + (size_t)cur * (n + 1);
-  int nch = (n + FB_CHUNK - 1) / FB_CHUNK;
+  int nch = (n + 1 + FB_CHUNK - 1) / FB_CHUNK;
   float *Rc = (float *)malloc(sizeof(float) * (size_t)nch);
   float *Cc = (float *)malloc(sizeof(float) * (size_t)nch);
   float lanes[64], P[64], w[64][FB_V];
@@ -343,7 +344,7 @@ static int fallback_select(const OracleColony *o, const uint16_t *state, int cur
     for (int l = 0; l < 64; ++l) {
       float s = 0.0f;
       for (int i = 0; i < FB_V; ++i) {
-        int j = 1 + c * FB_CHUNK + l * FB_V + i;
+        int j = c * FB_CHUNK + l * FB_V + i;
         float wt = 0.0f;
         if (j <= n && (int)state[j] <= room) {
           int d = oracle_nint_dist(o->x[cur], o->y[cur], o->x[j], o->y[j]);
@@ -375,7 +376,7 @@ static int fallback_select(const OracleColony *o, const uint16_t *state, int cur
   for (int l = 0; l < 64; ++l) {
     float s = 0.0f;
     for (int i = 0; i < FB_V; ++i) {
-      int j = 1 + cs * FB_CHUNK + l * FB_V + i;
+      int j = cs * FB_CHUNK + l * FB_V + i;
       float wt = 0.0f;
       if (j <= n && (int)state[j] <= room) {
         int d = oracle_nint_dist(o->x[cur], o->y[cur], o->x[j], o->y[j]);
@@ -404,7 +405,7 @@ static int fallback_select(const OracleColony *o, const uint16_t *state, int cur
     for (int i = FB_V - 1; i >= 0; --i)
       if (w[ls][i] > 0.0f) { is = i; break; }
   free(Rc); free(Cc);
-  return 1 + cs * FB_CHUNK + ls * FB_V + is;
+  return cs * FB_CHUNK + ls * FB_V + is;
 }
 
 int oracle_construct_ant(const OracleColony *o, int ant, int epoch, int32_t *tour,

@@ -35,7 +35,30 @@ CASES = {
     # full coordinate range: exercises the nint square root near 2*32767^2
     "wide_coords": (lambda: Instance.uniform(200, 4, side=32767, seed=5),
                     dict(cand_k=24, ants=32, rho=0.3)),
+    # short candidate lists: most late decisions take the all-customer
+    # fallback, over three 1024-node chunks
+    "n2100_k6_fallbacks": (lambda: Instance.uniform(2100, 12, seed=21),
+                           dict(cand_k=6, ants=32, rho=0.1)),
 }
+
+
+def nint_matrix(inst):
+    """nint(d(u, v)) for every node pair (sqrt of an integer is never a
+    half-integer, so rounding the double square root is exact)."""
+    x = np.asarray(inst.x, np.int64)
+    y = np.asarray(inst.y, np.int64)
+    s = (x[:, None] - x[None, :]) ** 2 + (y[:, None] - y[None, :]) ** 2
+    return np.rint(np.sqrt(s.astype(np.float64))).astype(np.int64)
+
+
+def check_fallback_weights(s, o, inst):
+    """The all-pairs fallback weights equal taupow[tau[u][v]] * eta[d(u,v)]
+    (one fp32 multiply) for the oracle's pheromone matrix."""
+    t = o.tables()
+    tau, _ = o.tau()
+    want = (t["taupow"][tau] * t["eta"][nint_matrix(inst)]).astype(np.float32)
+    got = s.fallback_weights()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
 def make_pair(name, **over):
@@ -87,10 +110,12 @@ def _compare_tours(s, o):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("stage", [1, 0])
-def test_construct_matches_oracle(gpu, name, stage):
-    inst, s, o = make_pair(name, stage_lds=stage)
+@pytest.mark.parametrize("stage,fm", [(1, 1), (0, 1), (1, 0)])
+def test_construct_matches_oracle(gpu, name, stage, fm):
+    inst, s, o = make_pair(name, stage_lds=stage, fallback_matrix=fm)
     s.prepare()
+    if fm:
+        check_fallback_weights(s, o, inst)
     s.construct()
     o.construct()
     gt, gl, gc, gu = _compare_tours(s, o)
@@ -104,9 +129,11 @@ def test_construct_matches_oracle(gpu, name, stage):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["c1_n100", "n300_clustered", "smoke_unit", "n1030_two_chunks"])
-def test_epochs_match_oracle(gpu, name):
-    inst, s, o = make_pair(name)
+@pytest.mark.parametrize("name,fm", [("c1_n100", 1), ("n300_clustered", 1), ("smoke_unit", 1),
+                                     ("n1030_two_chunks", 1), ("n1030_two_chunks", 0),
+                                     ("n2100_k6_fallbacks", 1)])
+def test_epochs_match_oracle(gpu, name, fm):
+    inst, s, o = make_pair(name, fallback_matrix=fm)
     s.prepare()
     for e in range(6):
         s.epochs(1)
@@ -125,6 +152,8 @@ def test_epochs_match_oracle(gpu, name):
         _, dist, _ = o.candidates()
         want = t["taupow"][co] * t["eta"][dist]
         assert np.array_equal(s.scores().view(np.uint32), want.astype(np.float32).view(np.uint32))
+        if fm:
+            check_fallback_weights(s, o, inst)
     if o.best()[1] is not None:
         bt, bc = s.best()
         ot, oc = o.best()
@@ -170,12 +199,14 @@ def test_deposit_tour_conflicts(gpu):
     tau = rng.integers(0, 256, size=(inst.n + 1, inst.n + 1), dtype=np.uint8)
     s.set_tau(tau)
     o.set_tau(tau)
+    check_fallback_weights(s, o, inst)
     tour = [0, 5, 0, 0, 7, 9, 11, 0, 0, 13, 0, 2, 3, 0]
     for delta in (0.0013, 0.05, 3.0):
         s.deposit_tour(tour, delta)
         o.deposit_tour(tour, delta)
         assert np.array_equal(s.tau()[0], o.tau()[0])
         assert np.array_equal(s.tau()[1], o.tau()[1])
+        check_fallback_weights(s, o, inst)
     s.close()
 
 

@@ -38,7 +38,8 @@ class AcoHipParams(C.Structure):
     _fields_ = [("cand_k", C.c_int), ("ants", C.c_int), ("colonies", C.c_int),
                 ("alpha", C.c_float), ("beta", C.c_float), ("rho", C.c_float),
                 ("tau0", C.c_float), ("Q", C.c_float), ("depot_close_weight", C.c_float),
-                ("knn_key", C.c_int), ("ants_per_group", C.c_int), ("stage_lds", C.c_int)]
+                ("knn_key", C.c_int), ("ants_per_group", C.c_int), ("stage_lds", C.c_int),
+                ("fallback_matrix", C.c_int)]
 
 
 class AcoHipColonyState(C.Structure):
@@ -71,7 +72,10 @@ class AcoHipTiming(C.Structure):
 class AcoHipStats(C.Structure):
     _fields_ = [("decisions", C.c_uint64), ("fallbacks", C.c_uint64),
                 ("fallback_cycles", C.c_uint64), ("ant_cycles", C.c_uint64),
-                ("ants", C.c_uint64), ("staging_cycles", C.c_uint64)]
+                ("ants", C.c_uint64), ("staging_cycles", C.c_uint64),
+                ("updates", C.c_uint64), ("evap_ticks", C.c_uint64),
+                ("select_ticks", C.c_uint64), ("deposit_ticks", C.c_uint64),
+                ("wall_clock_hz", C.c_uint64)]
 
 
 class VrpInstance(C.Structure):
@@ -113,6 +117,7 @@ HIP_SYMBOLS = {
     "aco_hip_get_tau": (C.c_int, [ctx_p, C.c_int, u8p, u8p]),
     "aco_hip_set_tau": (C.c_int, [ctx_p, C.c_int, u8p]),
     "aco_hip_get_scores": (C.c_int, [ctx_p, C.c_int, f32p]),
+    "aco_hip_get_fallback_weights": (C.c_int, [ctx_p, C.c_int, f32p]),
     "aco_hip_get_tours": (C.c_int, [ctx_p, C.c_int, i32p, i32p, i64p, i32p]),
     "aco_hip_get_colony": (C.c_int, [ctx_p, C.c_int, C.POINTER(AcoHipColonyState)]),
     "aco_hip_set_epoch": (C.c_int, [ctx_p, C.c_int, C.c_int]),

@@ -38,6 +38,8 @@ struct AcoHipContext {
   bool loaded = false;
   AcoHipParams prm{};
   int num_inst = 0, num_col = 0, W = 4, stage = 1, groups = 1, total_rows = 0, eta_lds = 1;
+  int rows = 1;        /* 16-lane rows of the widest candidate list (k_construct ROWS) */
+  int fs_on = 0;       /* all-pairs fallback weight matrix allocated */
   size_t lds_bytes = 0;
   std::vector<InstDesc> inst;
   std::vector<ColonyDesc> col;
@@ -46,8 +48,8 @@ struct AcoHipContext {
   std::vector<float> eta_lut;
   int eta_len = 0;
   size_t pher_total = 0, tours_total = 0, best_total = 0, rec_total = 0, nodes_total = 0;
-  size_t score_total = 0, slot_total = 0;
-  int ants_total = 0, max_stride = 0, max_mat_vec = 0, max_slots = 0;
+  size_t score_total = 0, slot_total = 0, fs_total = 0;
+  int ants_total = 0, max_stride = 0, max_mat_vec = 0, max_slots = 0, max_fs_items = 0;
   /* device pools */
   InstDesc *d_inst = nullptr;
   ColonyDesc *d_col = nullptr;
@@ -60,6 +62,7 @@ struct AcoHipContext {
   uint16_t *d_cidx = nullptr;
   uint8_t *d_slot = nullptr;
   float *d_score = nullptr;
+  float *d_fscore = nullptr;
   uint8_t *d_pher = nullptr;
   int32_t *d_tours = nullptr, *d_len = nullptr, *d_unvis = nullptr, *d_best = nullptr;
   long long *d_cost = nullptr;
@@ -128,7 +131,8 @@ static void free_pools(AcoHipContext *c) {
   void *ptrs[] = {c->d_inst, c->d_col,   c->d_state,  c->d_row_inst, c->d_xy,   c->d_dem,
                   c->d_dscore, c->d_rec, c->d_pher,   c->d_tours,    c->d_len,  c->d_unvis,
                   c->d_best, c->d_cost,  c->d_deq,    c->d_thr,      c->d_taupow, c->d_eta,
-                  c->d_scratch_tour, c->d_hist, c->d_stats, c->d_cidx, c->d_score, c->d_done, c->d_slot};
+                  c->d_scratch_tour, c->d_hist, c->d_stats, c->d_cidx, c->d_score, c->d_done, c->d_slot,
+                  c->d_fscore};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_inst = nullptr; c->d_col = nullptr; c->d_state = nullptr; c->d_row_inst = nullptr;
@@ -137,7 +141,7 @@ static void free_pools(AcoHipContext *c) {
   c->d_best = nullptr; c->d_cost = nullptr; c->d_deq = nullptr; c->d_thr = nullptr;
   c->d_taupow = nullptr; c->d_eta = nullptr; c->d_scratch_tour = nullptr; c->d_hist = nullptr;
   c->d_stats = nullptr; c->d_cidx = nullptr; c->d_score = nullptr; c->d_done = nullptr;
-  c->d_slot = nullptr;
+  c->d_slot = nullptr; c->d_fscore = nullptr;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -145,13 +149,21 @@ static void free_pools(AcoHipContext *c) {
   c->loaded = false;
 }
 
-/* LDS bytes of one construct workgroup for an instance of n1 nodes. */
-static size_t lds_for(int n1, int k, int eta_len, int W, int stage) {
-  size_t b = 256 * 4 + r16((size_t)eta_len * 4) + (size_t)W * r16((size_t)n1 * 2);
-  if (eta_len == 0) b = 256 * 4 + (size_t)W * r16((size_t)n1 * 2); /* eta table in global */
-  if (stage)
-    b += r16((size_t)n1 * k * 2) + r16((size_t)n1 * k * 4) + r16((size_t)n1 * 4) +
-         r16((size_t)n1 * 8);
+/* Row stride S of the all-pairs weight matrix and length of the per-ant
+ * state array: n+1 rounded up to 16. */
+static int fs_stride_of(int n1) { return (n1 + 15) & ~15; }
+
+/* LDS bytes of one construct workgroup for an instance of n1 nodes (layout
+ * of k_construct): staged candidate rows (+64 padding entries), depot scores
+ * and coordinates; the computed-fallback tables (taupow, and eta unless
+ * eta_len == 0) when the weight matrix is off; W state arrays of S u16. */
+static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on) {
+  size_t b = (size_t)W * (size_t)fs_stride_of(n1) * 2;
+  if (!fs_on) b += 256 * 4 + r16((size_t)eta_len * 4);
+  if (stage) {
+    const size_t ent = (size_t)n1 * k + 64;
+    b += r16(ent * 2) + r16(ent * 4) + r16((size_t)n1 * 4) + r16((size_t)n1 * 8);
+  }
   return b;
 }
 
@@ -315,9 +327,11 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     d.slot_off = (long long)slot_off;
     slot_off += r256(((size_t)a.n + 1) * ((size_t)a.n + 1));
     d.seed = a.seed;
+    d.fs_stride = fs_stride_of(a.n + 1);
     c->inst_n1[i] = a.n + 1;
     node_off += ((size_t)a.n + 1 + 3) & ~(size_t)3;
-    cand_off += (((size_t)a.n + 1) * d.k + 7) & ~(size_t)7;
+    /* 64 padding entries: every lane of a wavefront reads row n in bounds */
+    cand_off += (((size_t)a.n + 1) * d.k + 64 + 7) & ~(size_t)7;
     row_base += a.n + 1;
   }
   c->nodes_total = node_off;
@@ -333,9 +347,9 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
 
   /* colonies */
   c->col.assign(c->num_col, ColonyDesc{});
-  size_t pher = 0, tours = 0, best = 0, score = 0;
+  size_t pher = 0, tours = 0, best = 0, score = 0, fs = 0;
   int ants = 0, max_stride = 0;
-  size_t max_vec = 0, max_slots = 0;
+  size_t max_vec = 0, max_slots = 0, max_items = 0;
   for (int i = 0; i < num; ++i) {
     const InstDesc &d = c->inst[i];
     const size_t n1 = (size_t)d.n + 1;
@@ -347,15 +361,20 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       cd.m = p->ants;
       cd.stride = stride;
       cd.pher_off = (long long)pher;
-      cd.tcand_off = (long long)(pher + r16(n1 * n1));
-      cd.pher_bytes = (long long)(r16(n1 * n1) + r16(n1 * d.k));
+      /* pheromone rows padded to the stride S (16-byte aligned rows) */
+      const size_t mat = n1 * (size_t)d.fs_stride;
+      cd.tcand_off = (long long)(pher + mat);
+      cd.pher_bytes = (long long)(mat + r16(n1 * d.k));
       cd.tour_off = (long long)tours;
       cd.ant_off = ants;
       cd.best_off = (int)best;
       cd.score_off = (long long)score;
-      cd.mat_bytes = (long long)r16(n1 * n1);
-      score += (n1 * d.k + 3) & ~(size_t)3;
+      cd.mat_bytes = (long long)mat;
+      cd.fs_off = (long long)fs;
+      fs += n1 * (size_t)d.fs_stride;
+      score += (n1 * d.k + 64 + 3) & ~(size_t)3;
       max_slots = std::max(max_slots, n1 * d.k);
+      max_items = std::max(max_items, n1 * (size_t)(d.fs_stride / 16));
       pher += r256((size_t)cd.pher_bytes);
       tours += (size_t)p->ants * stride;
       best += stride;
@@ -371,7 +390,16 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   c->max_stride = max_stride;
   c->max_mat_vec = (int)std::min(max_vec, (size_t)1 << 30);
   c->max_slots = (int)std::min(max_slots, (size_t)1 << 30);
+  c->max_fs_items = (int)std::min(max_items, (size_t)1 << 30);
   c->score_total = score;
+  c->fs_total = fs;
+
+  /* all-pairs fallback weights: on unless disabled or above 16 GiB */
+  int want_fs = p->fallback_matrix;
+  if (want_fs < 0) want_fs = env_int("ACO_HIP_FALLBACK_MATRIX", -1);
+  const size_t fs_bytes = sizeof(float) * fs;
+  if (want_fs < 0) want_fs = fs_bytes <= ((size_t)16 << 30) ? 1 : 0;
+  c->fs_on = want_fs ? 1 : 0;
 
   /* launch shape: ants per workgroup and LDS staging */
   int W = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
@@ -383,11 +411,10 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   for (int i = 0; i < num; ++i) {
     max_n1 = std::max(max_n1, c->inst_n1[i]);
     max_k = std::max(max_k, c->inst[i].k);
-    need_stage = std::max(need_stage, lds_for(c->inst_n1[i], c->inst[i].k, c->eta_len, W, 1));
-    need_plain = std::max(need_plain, lds_for(c->inst_n1[i], c->inst[i].k, c->eta_len, W, 0));
   }
   (void)need_stage;
   (void)need_plain;
+  c->rows = (max_k + 15) / 16;
   int stage = want_stage == 0 ? 0 : 1;
   int eta_in_lds = 1;
   size_t need = 0;
@@ -395,7 +422,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     need = 0;
     for (int i = 0; i < num; ++i)
       need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0,
-                                    W, stage));
+                                    W, stage, c->fs_on));
   };
   /* preference: staged rows with the eta table, staged rows without it, then
    * (unless staging was forced) global rows, then fewer ants per workgroup */
@@ -437,13 +464,15 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     return rc;
   if ((rc = dalloc(c, &c->d_done, sizeof(int) * c->num_col))) return rc;
   if ((rc = dalloc(c, &c->d_row_inst, sizeof(int) * (size_t)c->total_rows))) return rc;
-  if ((rc = dalloc(c, &c->d_xy, sizeof(int2) * c->nodes_total))) return rc;
+  /* +16 entries: 16-node vector reads of the last instance stay in bounds */
+  if ((rc = dalloc(c, &c->d_xy, sizeof(int2) * (c->nodes_total + 16)))) return rc;
   if ((rc = dalloc(c, &c->d_dem, sizeof(uint16_t) * c->nodes_total))) return rc;
   if ((rc = dalloc(c, &c->d_dscore, sizeof(float) * c->nodes_total))) return rc;
   if ((rc = dalloc(c, &c->d_rec, sizeof(uint32_t) * c->rec_total))) return rc;
   if ((rc = dalloc(c, &c->d_cidx, sizeof(uint16_t) * c->rec_total))) return rc;
   if ((rc = dalloc(c, &c->d_slot, c->slot_total))) return rc;
   if ((rc = dalloc(c, &c->d_score, sizeof(float) * c->score_total))) return rc;
+  if (c->fs_on && (rc = dalloc(c, &c->d_fscore, sizeof(float) * c->fs_total))) return rc;
   if ((rc = dalloc(c, &c->d_pher, c->pher_total))) return rc;
   if ((rc = dalloc(c, &c->d_tours, sizeof(int32_t) * c->tours_total))) return rc;
   if ((rc = dalloc(c, &c->d_len, sizeof(int32_t) * (size_t)ants))) return rc;
@@ -492,6 +521,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.cidx = c->d_cidx;
   P.slot_of = c->d_slot;
   P.score = c->d_score;
+  P.fscore = c->fs_on ? c->d_fscore : nullptr;
   P.pher = c->d_pher;
   P.tours = c->d_tours;
   P.len = c->d_len;
@@ -514,10 +544,11 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.stats = nullptr;
   P.done = c->d_done;
 
-  const void *variants[] = {(const void *)k_construct<true, false>,
-                            (const void *)k_construct<false, false>,
-                            (const void *)k_construct<true, true>,
-                            (const void *)k_construct<false, true>};
+#define ACO_VARIANTS(R)                                                                         \
+  (const void *)k_construct<true, false, R>, (const void *)k_construct<false, false, R>,        \
+      (const void *)k_construct<true, true, R>, (const void *)k_construct<false, true, R>
+  const void *variants[] = {ACO_VARIANTS(1), ACO_VARIANTS(2), ACO_VARIANTS(3), ACO_VARIANTS(4)};
+#undef ACO_VARIANTS
   for (const void *f : variants)
     HIPC(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
   c->loaded = true;
@@ -583,23 +614,33 @@ extern "C" int aco_hip_init_tau(AcoHipContext *c) {
   return ACO_HIP_OK;
 }
 
+template <int R>
+static void launch_construct_rows(AcoHipContext *c, dim3 grid, dim3 block, bool stats) {
+  if (c->stage && !stats)
+    hipLaunchKernelGGL((k_construct<true, false, R>), grid, block, c->lds_bytes, c->stream, c->P,
+                       c->groups, c->W);
+  else if (!stats)
+    hipLaunchKernelGGL((k_construct<false, false, R>), grid, block, c->lds_bytes, c->stream, c->P,
+                       c->groups, c->W);
+  else if (c->stage)
+    hipLaunchKernelGGL((k_construct<true, true, R>), grid, block, c->lds_bytes, c->stream, c->P,
+                       c->groups, c->W);
+  else
+    hipLaunchKernelGGL((k_construct<false, true, R>), grid, block, c->lds_bytes, c->stream, c->P,
+                       c->groups, c->W);
+}
+
 static int launch_construct(AcoHipContext *c) {
   {
     Timed tm(c, T_CONSTRUCT);
     dim3 grid((unsigned)(c->num_col * c->groups)), block((unsigned)(64 * c->W));
     const bool stats = c->P.stats != nullptr;
-    if (c->stage && !stats)
-      hipLaunchKernelGGL((k_construct<true, false>), grid, block, c->lds_bytes, c->stream, c->P,
-                         c->groups, c->W);
-    else if (!stats)
-      hipLaunchKernelGGL((k_construct<false, false>), grid, block, c->lds_bytes, c->stream, c->P,
-                         c->groups, c->W);
-    else if (c->stage)
-      hipLaunchKernelGGL((k_construct<true, true>), grid, block, c->lds_bytes, c->stream, c->P,
-                         c->groups, c->W);
-    else
-      hipLaunchKernelGGL((k_construct<false, true>), grid, block, c->lds_bytes, c->stream, c->P,
-                         c->groups, c->W);
+    switch (c->rows) {
+      case 1: launch_construct_rows<1>(c, grid, block, stats); break;
+      case 2: launch_construct_rows<2>(c, grid, block, stats); break;
+      case 3: launch_construct_rows<3>(c, grid, block, stats); break;
+      default: launch_construct_rows<4>(c, grid, block, stats); break;
+    }
   }
   LAUNCHED(c, "k_construct");
   return ACO_HIP_OK;
@@ -615,7 +656,8 @@ static int launch_update(AcoHipContext *c, int mode) {
 }
 
 static int launch_evap(AcoHipContext *c, int from_ants) {
-  const int xm = std::max(1, std::min((c->max_mat_vec + 255) / 256, 1024));
+  const int mat = c->fs_on ? c->max_fs_items : c->max_mat_vec;
+  const int xm = std::max(1, std::min((mat + 255) / 256, 1024));
   const int xs = std::max(1, std::min((c->max_slots + 255) / 256, 512));
   {
     Timed tm(c, T_EVAP);
@@ -627,7 +669,8 @@ static int launch_evap(AcoHipContext *c, int from_ants) {
 }
 
 static int launch_evap_update(AcoHipContext *c) {
-  const int xm = std::max(1, std::min((c->max_mat_vec + 1023) / 1024, 512));
+  const int mat = c->fs_on ? c->max_fs_items : c->max_mat_vec;
+  const int xm = std::max(1, std::min((mat + 1023) / 1024, 512));
   const int xs = std::max(1, std::min((c->max_slots + 1023) / 1024, 256));
   {
     Timed tm(c, T_EVAP);
@@ -639,7 +682,8 @@ static int launch_evap_update(AcoHipContext *c) {
 }
 
 static int launch_refresh(AcoHipContext *c) {
-  const int xs = std::max(1, std::min((c->max_slots + 255) / 256, 512));
+  const int work = std::max(c->max_slots, c->fs_on ? c->max_fs_items : 0);
+  const int xs = std::max(1, std::min((work + 255) / 256, 1024));
   hipLaunchKernelGGL(k_refresh_scores, dim3(xs, c->num_col), dim3(256), 0, c->stream, c->P);
   LAUNCHED(c, "k_refresh_scores");
   return ACO_HIP_OK;
@@ -867,7 +911,9 @@ extern "C" int aco_hip_get_tau(AcoHipContext *c, int colony, uint8_t *tau, uint8
   const InstDesc &d = c->inst[cd.inst];
   const size_t n1 = (size_t)d.n + 1;
   HIPC(c, hipStreamSynchronize(c->stream));
-  if (tau) HIPC(c, hipMemcpy(tau, c->d_pher + cd.pher_off, n1 * n1, hipMemcpyDeviceToHost));
+  if (tau)
+    HIPC(c, hipMemcpy2D(tau, n1, c->d_pher + cd.pher_off, (size_t)d.fs_stride, n1, n1,
+                        hipMemcpyDeviceToHost));
   if (tau_cand)
     HIPC(c, hipMemcpy(tau_cand, c->d_pher + cd.tcand_off, n1 * d.k, hipMemcpyDeviceToHost));
   return ACO_HIP_OK;
@@ -887,7 +933,8 @@ extern "C" int aco_hip_set_tau(AcoHipContext *c, int colony, const uint8_t *tau)
   HIPC(c, hipMemcpy(rec.data(), c->d_rec + d.cand_off, cnt * 4, hipMemcpyDeviceToHost));
   for (size_t i = 0; i < n1; ++i)
     for (int t = 0; t < d.k; ++t) tc[i * d.k + t] = tau[i * n1 + (rec[i * d.k + t] & 0xFFFFu)];
-  HIPC(c, hipMemcpy(c->d_pher + cd.pher_off, tau, n1 * n1, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy2D(c->d_pher + cd.pher_off, (size_t)d.fs_stride, tau, n1, n1, n1,
+                      hipMemcpyHostToDevice));
   HIPC(c, hipMemcpy(c->d_pher + cd.tcand_off, tc.data(), cnt, hipMemcpyHostToDevice));
   int rc2 = launch_refresh(c);
   if (rc2) return rc2;
@@ -905,6 +952,22 @@ extern "C" int aco_hip_get_scores(AcoHipContext *c, int colony, float *score) {
   HIPC(c, hipStreamSynchronize(c->stream));
   HIPC(c, hipMemcpy(score, c->d_score + cd.score_off, sizeof(float) * (size_t)(d.n + 1) * d.k,
                     hipMemcpyDeviceToHost));
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_get_fallback_weights(AcoHipContext *c, int colony, float *weights) {
+  NEED_LOADED(c);
+  int rc;
+  if ((rc = check_col(c, colony))) return rc;
+  if (!weights) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null weights");
+  if (!c->fs_on) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "fallback weight matrix is off");
+  const ColonyDesc &cd = c->col[colony];
+  const InstDesc &d = c->inst[cd.inst];
+  const size_t n1 = (size_t)d.n + 1;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy2D(weights, n1 * sizeof(float), c->d_fscore + cd.fs_off,
+                      (size_t)d.fs_stride * sizeof(float), n1 * sizeof(float), n1,
+                      hipMemcpyDeviceToHost));
   return ACO_HIP_OK;
 }
 
@@ -1058,6 +1121,13 @@ extern "C" int aco_hip_get_stats(AcoHipContext *c, int colony, AcoHipStats *st) 
   st->ant_cycles = v[3];
   st->ants = v[4];
   st->staging_cycles = v[5];
+  st->updates = v[6];
+  st->evap_ticks = v[7];
+  st->select_ticks = v[8];
+  st->deposit_ticks = v[9];
+  int khz = 0;
+  HIPC(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+  st->wall_clock_hz = (uint64_t)khz * 1000u;
   return ACO_HIP_OK;
 }
 

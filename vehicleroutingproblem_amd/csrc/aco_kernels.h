@@ -6,12 +6,18 @@
  *   xy      int2   [sum(n_i+1) padded]   integer coordinates per node
  *   dem     uint16 [same]                demand per node
  *   dscore  float  [same]                depot-close score 2*(1/d(i,0))^beta
- *   rec     uint32 [sum((n_i+1)*k)]      candidate records idx | nint(d) << 16
- *   cidx    uint16 [same]                candidate indices (rec & 0xFFFF)
+ *   rec     uint32 [sum((n_i+1)*k + 64)] candidate records idx | nint(d) << 16
+ *   cidx    uint16 [same]                candidate indices (rec & 0xFFFF); 64 zero
+ *                  entries after each instance's table let every lane of a
+ *                  wavefront read row cur in bounds
  *   slot_of uint8  per instance (n+1)^2   1 + slot of v among u's candidates, 0 if
  *                  v is not one (lets a deposit patch tau_cand in one load)
- *   score   float  per colony (n+1)*k    taupow[tau_cand] * eta[d], refreshed by
+ *   score   float  per colony (n+1)*k+64 taupow[tau_cand] * eta[d], refreshed by
  *                  the evaporation kernel and patched by the deposits
+ *   fscore  float  per colony (n+1) rows x S (S = n+1 rounded up to 16):
+ *                  taupow[tau[u][v]] * eta[d(u,v)] for every pair, the weights
+ *                  of the all-customer fallback roulette, kept in step with tau
+ *                  by the same kernels (optional: NULL when too large)
  *   pher    uint8  per colony: (n+1)^2 pheromone matrix (row stride n+1,
  *                  the reference format), then the (n+1)*k candidate-aligned
  *                  copy tau_cand[i][t] == tau[i][cand[i][t]], 16-byte padded
@@ -37,7 +43,8 @@ struct InstDesc {
   long long cand_off;  /* into rec and cidx (multiple of 8) */
   long long slot_off;  /* bytes into slot_of: (n+1)^2, slot+1 of v in row u or 0 */
   uint32_t seed;
-  int pad;
+  int fs_stride;       /* S = n+1 rounded up to 16: fscore row stride and the
+                          per-ant state length in LDS */
 };
 
 struct ColonyDesc {
@@ -53,6 +60,7 @@ struct ColonyDesc {
   int best_off;        /* int32 elements into best */
   long long score_off; /* float elements into score (multiple of 4) */
   long long mat_bytes; /* (n+1)^2 rounded to 16: the matrix part of the block */
+  long long fs_off;    /* float elements into fscore (multiple of 16) */
 };
 
 struct DevCtx {
@@ -67,6 +75,7 @@ struct DevCtx {
   uint16_t *cidx;      /* candidate index per slot (rec & 0xFFFF), per instance */
   uint8_t *slot_of;    /* per instance (n+1)^2: 1 + slot of v in row u, 0 if none */
   float *score;        /* taupow[tau_cand] * eta[d] per slot, per colony */
+  float *fscore;       /* per colony (n+1) x S all-pairs weights, NULL = off */
   uint8_t *pher;
   int32_t *tours;
   int32_t *len;
@@ -90,9 +99,12 @@ struct DevCtx {
   int *done;           /* per-colony block counter of k_evap_update (kept at 0) */
 };
 
-/* construct counters per colony: decisions, fallbacks, fallback cycles,
- * ant cycles, ants (clock64 = shader clock) */
-#define ACO_STATS 8
+/* counters per colony: construct [0] decisions, [1] fallbacks, [2] fallback
+ * cycles, [3] ant cycles, [4] ants, [5] staging cycles (clock64 = shader
+ * clock); update kernel [6] updates, [7] evaporation ticks, [8] selection
+ * ticks, [9] deposit ticks (wall_clock64), [10] ~(first block start) of the
+ * running update (reset to 0 by its last block) */
+#define ACO_STATS 16
 
 #define ACO_HIST 64
 struct HistEntry {
@@ -117,7 +129,7 @@ static inline __host__ __device__ int aco_significant(double prev, double now, d
 /* kernels (aco_kernels.hip) */
 __global__ void k_knn(DevCtx P, int total_rows, int knn_key, float depot_w);
 __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
-template <bool STAGE, bool STATS>
+template <bool STAGE, bool STATS, int ROWS>
 __global__ void k_construct(DevCtx P, int groups_per_colony, int W);
 __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
 __global__ void k_refresh_scores(DevCtx P);

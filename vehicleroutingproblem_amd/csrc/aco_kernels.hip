@@ -197,55 +197,83 @@ struct FbOut {
   float T;
 };
 
-/* Weights of the 16 customers j0..j0+15 of one lane (0 when not eligible).
- * Every load is issued before any is used and the ineligible weights are
- * zeroed by a multiply (weights are finite), so no lane branches. */
-__device__ __forceinline__ float fb_lane_weights(float (&w)[FB_V], int j0, int n, int room,
-                                                 int2 pc, const int2 *xy, const float *taupow,
-                                                 const float *eta, const uint16_t *state,
-                                                 const uint8_t *trow) {
+/* The fallback roulette partitions the nodes 0..n into chunks of 1024:
+ * node j = c*1024 + l*16 + i belongs to chunk c, lane l, slot i (the depot is
+ * visited from the start, so it never carries weight).  A lane's 16 nodes are
+ * contiguous, and so are its fallback weights and states. */
+
+/* Weights of the 16 nodes b..b+15 of one lane from the all-pairs weight row
+ * (0 when visited, too heavy, or past the last node).  Lanes past the last
+ * chunk read node 0's group and keep nothing. */
+__device__ __forceinline__ float fb_lane_fs(float (&w)[FB_V], int b0, int n1, int room,
+                                            const float *__restrict__ frow,
+                                            const uint16_t *state) {
+  const bool live = b0 < n1;
+  const int b = live ? b0 : 0;
+  const float4 *f4 = reinterpret_cast<const float4 *>(frow + b);
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(state + b);
+  const float4 fa = f4[0], fb = f4[1], fc = f4[2], fd = f4[3];
+  const uint4 sa = s4[0], sb = s4[1];
+  const float fv[FB_V] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w,
+                          fc.x, fc.y, fc.z, fc.w, fd.x, fd.y, fd.z, fd.w};
+  const uint32_t sw[8] = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y, sb.z, sb.w};
+  const int lim = live ? room : -1;
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < FB_V; ++i) {
+    const int st = (int)((sw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
+    w[i] = st <= lim ? fv[i] : 0.0f;
+    s = s + w[i];
+  }
+  return s;
+}
+
+/* Same weights computed from the pheromone row, the coordinates and the
+ * taupow / eta tables (used when the all-pairs weight matrix is off). */
+__device__ __forceinline__ float fb_lane_tables(float (&w)[FB_V], int b0, int n1, int room,
+                                                int2 pc, const int2 *xy, const float *taupow,
+                                                const float *eta, const uint16_t *state,
+                                                const uint8_t *__restrict__ trow) {
   uint32_t st[FB_V], tq[FB_V];
   int2 pj[FB_V];
 #pragma unroll
   for (int i = 0; i < FB_V; ++i) {
-    const int j = j0 + i;
-    const int jc = j <= n ? j : n;
+    const int j = b0 + i;
+    const int jc = j < n1 ? j : 0;
     st[i] = state[jc];
     tq[i] = trow[jc];
     pj[i] = xy[jc];
   }
-  float tp[FB_V], et[FB_V];
-#pragma unroll
-  for (int i = 0; i < FB_V; ++i) {
-    const int d = aco_nint_dist(pc.x, pc.y, pj[i].x, pj[i].y);
-    tp[i] = taupow[tq[i]];
-    et[i] = eta[d];
-  }
   float s = 0.0f;
 #pragma unroll
   for (int i = 0; i < FB_V; ++i) {
-    const float keep = (j0 + i <= n && (int)st[i] <= room) ? 1.0f : 0.0f;
-    w[i] = (tp[i] * et[i]) * keep;
+    const int d = aco_nint_dist(pc.x, pc.y, pj[i].x, pj[i].y);
+    const float v = taupow[tq[i]] * eta[d];
+    w[i] = (b0 + i < n1 && (int)st[i] <= room) ? v : 0.0f;
     s = s + w[i];
   }
   return s;
 }
 
 /* Fallback roulette over every unvisited customer that fits (the exact
- * two-pass restatement of select_global_fallback, aco_cuda_kernels.cu:336-430).
- * Customer j = 1 + c*1024 + lane*16 + i.  Returns the node (or -1), its demand
- * and the total weight. */
-__device__ __forceinline__ FbOut fallback_select(const int2 *xy, const float *taupow,
-                                                 const float *eta, const uint16_t *state,
-                                                 const uint8_t *__restrict__ trow, int n, int cur,
-                                                 int room, float u, int lane) {
-  const int nch = (n + FB_CHUNK - 1) / FB_CHUNK;
+ * two-pass restatement of select_global_fallback, aco_cuda_kernels.cu:336-430):
+ * chunk totals, then the chunk, the lane and the slot holding the threshold.
+ * frow != NULL selects the all-pairs weight row; otherwise the weights are
+ * computed from trow and the tables.  Returns the node (or -1), its demand and
+ * the total weight. */
+__device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow, const int2 *xy,
+                                              const float *taupow, const float *eta,
+                                              const uint16_t *state,
+                                              const uint8_t *__restrict__ trow, int n1, int cur,
+                                              int room, float u, int lane) {
+  const int nch = (n1 + FB_CHUNK - 1) / FB_CHUNK;
   const int2 pc = xy[cur];
   float w[FB_V];
   float Rv = 0.0f, Cv = 0.0f, R = 0.0f, s = 0.0f, Pl = 0.0f;
   for (int c = 0; c < nch; ++c) {
-    s = fb_lane_weights(w, 1 + c * FB_CHUNK + lane * FB_V, n, room, pc, xy, taupow, eta, state,
-                        trow);
+    const int b0 = c * FB_CHUNK + lane * FB_V;
+    s = frow ? fb_lane_fs(w, b0, n1, room, frow, state)
+             : fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow);
     Pl = wave_scan_rows(s, lane, 4);
     const float Cc = f_readlane(Pl, 63);
     R = R + Cc;
@@ -265,8 +293,9 @@ __device__ __forceinline__ FbOut fallback_select(const int2 *xy, const float *ta
   const float base = cs > 0 ? f_readlane(Rv, cs - 1) : 0.0f;
   const float thr_c = thr - base;
   if (cs != nch - 1) {
-    s = fb_lane_weights(w, 1 + cs * FB_CHUNK + lane * FB_V, n, room, pc, xy, taupow, eta, state,
-                        trow);
+    const int b0 = cs * FB_CHUNK + lane * FB_V;
+    s = frow ? fb_lane_fs(w, b0, n1, room, frow, state)
+             : fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow);
     Pl = wave_scan_rows(s, lane, 4);
   }
   const uint64_t lm = ballot64(s > 0.0f && Pl > thr_c);
@@ -285,16 +314,41 @@ __device__ __forceinline__ FbOut fallback_select(const int2 *xy, const float *ta
   }
   if (is < 0) is = il;
   const int isel = __builtin_amdgcn_readlane(is, ls);
-  out.move = 1 + cs * FB_CHUNK + ls * FB_V + isel;
+  out.move = cs * FB_CHUNK + ls * FB_V + isel;
   out.mdem = __builtin_amdgcn_readfirstlane((int)state[out.move]);
   return out;
 }
 
-/* One wavefront per ant.  Per decision: lane t < k reads candidate t of the
- * current node (index and precomputed score), then that node's state (its
- * demand, or VISITED); a wavefront scan, one ballot and one bit-scan select
- * the move.  All control flow is uniform (scalar). */
-template <bool STAGE, bool STATS>
+/* Inclusive scan of the candidate weights with the row order of
+ * wave_scan_rows, specialised for ROWS rows: row 1 takes lane 15 through a
+ * row_bcast:15 DPP add (the same single addition as a readlane carry). */
+template <int ROWS>
+__device__ __forceinline__ float cand_scan(float v, int lane) {
+  v = v + f_dpp_row_shr1(v);
+  v = v + f_dpp_row_shr2(v);
+  v = v + f_dpp_row_shr4(v);
+  v = v + f_dpp_row_shr8(v);
+  if constexpr (ROWS >= 2)
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0x2, 0xF,
+                                                       false));
+  if constexpr (ROWS >= 3) {
+    const float c = f_readlane(v, 31);
+    if ((lane >> 4) == 2) v = v + c;
+  }
+  if constexpr (ROWS >= 4) {
+    const float c = f_readlane(v, 47);
+    if ((lane >> 4) == 3) v = v + c;
+  }
+  return v;
+}
+
+/* One wavefront per ant.  Per decision every lane reads entry `lane` of the
+ * current node's candidate row (lanes >= k read past the row, in bounds by
+ * the 64-entry padding, and are masked out of the ballots), then that node's
+ * state (its demand, or VISITED); a DPP scan, two ballots and one bit-scan
+ * select the move.  All control flow is uniform (scalar).  ROWS = 16-lane rows
+ * the widest candidate list spans. */
+template <bool STAGE, bool STATS, int ROWS>
 __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colony, int W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const unsigned long long t_kernel = STATS ? clock64() : 0ull;
@@ -305,11 +359,13 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   if (S->halted) return; /* uniform over the workgroup */
   const int epoch = S->epoch;
   const InstDesc I = P.inst[D.inst];
-  const int n = I.n, k = I.k, K = I.K, Q = I.Q, n1 = n + 1;
+  const int n = I.n, k = I.k, K = I.K, Q = I.Q, n1 = n + 1, Sn = I.fs_stride;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  const bool fs_on = P.fscore != nullptr;
 
   /* ---- LDS carve-up and staging ---- */
+  const size_t ent = (size_t)n1 * k + 64; /* candidate entries incl. padding */
   size_t off = 0;
   const uint16_t *c_idx;
   const float *c_score;
@@ -317,15 +373,15 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   const int2 *c_xy;
   if (STAGE) {
     uint16_t *s_idx = reinterpret_cast<uint16_t *>(smem + off);
-    off += round16((size_t)n1 * k * 2);
+    off += round16(ent * 2);
     float *s_score = reinterpret_cast<float *>(smem + off);
-    off += round16((size_t)n1 * k * 4);
+    off += round16(ent * 4);
     float *s_dsc = reinterpret_cast<float *>(smem + off);
     off += round16((size_t)n1 * 4);
     int2 *s_xy = reinterpret_cast<int2 *>(smem + off);
     off += round16((size_t)n1 * 8);
-    wg_copy16(s_idx, P.cidx + I.cand_off, round16((size_t)n1 * k * 2));
-    wg_copy16(s_score, P.score + D.score_off, round16((size_t)n1 * k * 4));
+    wg_copy16(s_idx, P.cidx + I.cand_off, round16(ent * 2));
+    wg_copy16(s_score, P.score + D.score_off, round16(ent * 4));
     wg_copy16(s_dsc, P.dscore + I.node_off, round16((size_t)n1 * 4));
     wg_copy16(s_xy, P.xy + I.node_off, round16((size_t)n1 * 8));
     c_idx = s_idx;
@@ -338,19 +394,23 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
     c_dsc = P.dscore + I.node_off;
     c_xy = P.xy + I.node_off;
   }
-  float *s_taupow = reinterpret_cast<float *>(smem + off);
-  off += 256 * 4;
-  float *s_eta = reinterpret_cast<float *>(smem + off);
-  wg_copy16(s_taupow, P.taupow, 256 * 4);
-  if (P.eta_lds) {
-    off += round16((size_t)P.eta_len * 4);
-    wg_copy16(s_eta, P.eta, round16((size_t)P.eta_len * 4));
+  /* the tables of the computed fallback (only without the weight matrix);
+   * large coordinate ranges keep the eta table in global memory */
+  const float *c_taupow = P.taupow;
+  const float *c_eta = P.eta;
+  if (!fs_on) {
+    float *s_taupow = reinterpret_cast<float *>(smem + off);
+    off += 256 * 4;
+    wg_copy16(s_taupow, P.taupow, 256 * 4);
+    c_taupow = s_taupow;
+    if (P.eta_lds) {
+      float *s_eta = reinterpret_cast<float *>(smem + off);
+      off += round16((size_t)P.eta_len * 4);
+      wg_copy16(s_eta, P.eta, round16((size_t)P.eta_len * 4));
+      c_eta = s_eta;
+    }
   }
-  /* large coordinate ranges keep the eta table in global memory (only the
-   * fallback reads it) */
-  const float *c_eta = P.eta_lds ? (const float *)s_eta : P.eta;
-  const int state_stride = (int)round16((size_t)n1 * 2);
-  uint16_t *s_state = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * state_stride);
+  uint16_t *s_state = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * Sn * 2);
   __syncthreads();
   if (STATS && threadIdx.x == 0)
     atomicAdd(P.stats + (size_t)colony * ACO_STATS + 5, clock64() - t_kernel);
@@ -360,20 +420,20 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
 
   /* ---- reset (kernel_reset_ant_state, aco_cuda_kernels.cu:227-257): the
    * state of node j is its demand while unvisited, VISITED after; the depot
-   * starts visited ---- */
+   * and the padding up to S start visited ---- */
   const uint16_t *g_dem = P.dem + I.node_off;
-  for (int j = lane; j < n1; j += 64) s_state[j] = (j == 0) ? (uint16_t)VISITED : g_dem[j];
+  for (int j = lane; j < Sn; j += 64)
+    s_state[j] = (j == 0 || j >= n1) ? (uint16_t)VISITED : g_dem[j];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
   const uint8_t *__restrict__ tau = P.pher + D.pher_off;
+  const float *__restrict__ fsm = fs_on ? P.fscore + D.fs_off : nullptr;
   int32_t *tour_out = P.tours + D.tour_off + (size_t)ant * D.stride;
   const uint32_t key0 = I.seed;
   const uint32_t cid = (uint32_t)D.colony_in_inst;
-  const int rows = (k + 15) >> 4;
-  const int klane = lane < k ? lane : 0;
-  const bool in_k = lane < k;
+  const uint64_t kmask = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
 
   const unsigned long long t_start = STATS ? clock64() : 0ull;
   unsigned long long fallbacks = 0, fb_cycles = 0;
@@ -418,27 +478,32 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
         rf1 = aco_u01(r[1]);
       }
       ++s;
+      const float u0 = f_readlane(rf0, sl);
 
       /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322) */
-      const int e = cur * k + klane;
-      const int j = in_k ? (int)c_idx[e] : 0;
+      const int e = cur * k + lane;
+      const int j = (int)c_idx[e];
       const float sc = c_score[e];
       const float dsc_cur = c_dsc[cur]; /* issued with the first batch of loads */
       const int st = (int)s_state[j];
       const float w = (st <= room) ? sc : 0.0f;
-      const float Pf = wave_scan_rows(w, lane, rows);
-      float T = f_readlane(Pf, k - 1);
+      const uint64_t emask = ballot64(w > 0.0f) & kmask;
+      const float Pf = cand_scan<ROWS>(w, lane);
       int move, mdem;
-      if (T > 0.0f) {
-        const float thr = f_readlane(rf0, sl) * T;
-        const uint64_t m = ballot64(w > 0.0f && Pf > thr);
-        const int sel = m ? first_lane(m) : last_lane(ballot64(w > 0.0f));
+      float T;
+      if (__builtin_expect(emask != 0, 1)) {
+        /* T > 0 exactly when some weight is > 0 (sums of non-negatives) */
+        T = f_readlane(Pf, k - 1);
+        const float thr = u0 * T;
+        const uint64_t m = ballot64(Pf > thr) & emask;
+        const int sel = m ? first_lane(m) : last_lane(emask);
         move = __builtin_amdgcn_readlane(j, sel);
         mdem = __builtin_amdgcn_readlane(st, sel);
       } else {
         const unsigned long long f0 = STATS ? clock64() : 0ull;
-        const FbOut fb = fallback_select(c_xy, s_taupow, c_eta, s_state, tau + (size_t)cur * n1,
-                                         n, cur, room, f_readlane(rf0, sl), lane);
+        const FbOut fb = fallback_select(fsm ? fsm + (size_t)cur * Sn : nullptr, c_xy, c_taupow,
+                                         c_eta, s_state, tau + (size_t)cur * Sn, n1, cur, room,
+                                         u0, lane);
         move = fb.move;
         mdem = fb.mdem;
         T = fb.T;
@@ -490,10 +555,16 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   }
 }
 
-template __global__ void k_construct<true, false>(DevCtx, int, int);
-template __global__ void k_construct<false, false>(DevCtx, int, int);
-template __global__ void k_construct<true, true>(DevCtx, int, int);
-template __global__ void k_construct<false, true>(DevCtx, int, int);
+#define ACO_CONSTRUCT_VARIANTS(R)                                     \
+  template __global__ void k_construct<true, false, R>(DevCtx, int, int);  \
+  template __global__ void k_construct<false, false, R>(DevCtx, int, int); \
+  template __global__ void k_construct<true, true, R>(DevCtx, int, int);   \
+  template __global__ void k_construct<false, true, R>(DevCtx, int, int);
+ACO_CONSTRUCT_VARIANTS(1)
+ACO_CONSTRUCT_VARIANTS(2)
+ACO_CONSTRUCT_VARIANTS(3)
+ACO_CONSTRUCT_VARIANTS(4)
+#undef ACO_CONSTRUCT_VARIANTS
 
 /* ---- evaporation ---------------------------------------------------------- */
 
@@ -506,6 +577,57 @@ __device__ __forceinline__ uint32_t evap_word(uint32_t x, uint32_t d) {
     r |= q << (8 * b);
   }
   return r;
+}
+
+/* Items of the all-pairs weight matrix: item it covers row u = it / G and
+ * the 16 columns 16g..16g+15 (G = S/16 items per row).  fscore[u][v] =
+ * taupow[tau[u][v]] * eta[nint d(u,v)], 0 past the last node.  Pheromone and
+ * weight rows share the 16-aligned stride S, so an item is one 16-byte
+ * pheromone load (and, with evap, one evaporated 16-byte store: q > delta ?
+ * q - delta : 0 per byte) and four 16-byte weight stores.  `taupow` may be an
+ * LDS copy of the table.  Coordinates of a group are read as 16-byte pairs;
+ * the coordinate pool is padded by 16 entries past the last instance. */
+__device__ __forceinline__ void fs_items(const DevCtx &P, const ColonyDesc &D, const InstDesc &I,
+                                         const float *taupow, uint32_t first, uint32_t step,
+                                         bool evap, uint32_t dq) {
+  const int n1 = I.n + 1, S = I.fs_stride;
+  const uint32_t G = (uint32_t)S >> 4;
+  const uint32_t items = (uint32_t)n1 * G;
+  uint8_t *__restrict__ tau = P.pher + D.pher_off;
+  float *__restrict__ fs = P.fscore + D.fs_off;
+  const int2 *__restrict__ xy = P.xy + I.node_off;
+  for (uint32_t it = first; it < items; it += step) {
+    const uint32_t u = it / G;
+    const int v0 = (int)(it - u * G) * 16;
+    const size_t row = (size_t)u * S + v0;
+    uint4 t = *reinterpret_cast<const uint4 *>(tau + row);
+    if (evap) {
+      t.x = evap_word(t.x, dq);
+      t.y = evap_word(t.y, dq);
+      t.z = evap_word(t.z, dq);
+      t.w = evap_word(t.w, dq);
+      *reinterpret_cast<uint4 *>(tau + row) = t;
+    }
+    const int2 pu = xy[u];
+    const int4 *x4 = reinterpret_cast<const int4 *>(xy + v0);
+    int4 q[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) q[h] = x4[h];
+    const uint32_t tw[4] = {t.x, t.y, t.z, t.w};
+    float o[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int px = (i & 1) ? q[i >> 1].z : q[i >> 1].x;
+      const int py = (i & 1) ? q[i >> 1].w : q[i >> 1].y;
+      const uint32_t b = (tw[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+      const int d = aco_nint_dist(pu.x, pu.y, px, py);
+      o[i] = v0 + i < n1 ? taupow[b] * P.eta[v0 + i < n1 ? d : 0] : 0.0f;
+    }
+    float4 *f4 = reinterpret_cast<float4 *>(fs + row);
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+      f4[h] = make_float4(o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]);
+  }
 }
 
 /* Saturating subtract (kernel_evaporate_tau, aco_cuda_kernels.cu:145-155).
@@ -531,6 +653,11 @@ __global__ void __launch_bounds__(256) k_evaporate(DevCtx P, int mat_blocks, int
   }
   const uint32_t d = P.q_evap;
   if ((int)blockIdx.x < mat_blocks) {
+    if (P.fscore) {
+      fs_items(P, D, P.inst[D.inst], P.taupow, blockIdx.x * blockDim.x + threadIdx.x,
+               (uint32_t)mat_blocks * blockDim.x, true, d);
+      return;
+    }
     uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
     const size_t nvec = (size_t)D.mat_bytes / 16;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nvec;
@@ -559,8 +686,9 @@ __global__ void __launch_bounds__(256) k_evaporate(DevCtx P, int mat_blocks, int
   }
 }
 
-/* score[slot] = taupow[tau_cand[slot]] * eta[d(slot)] for every slot (after
- * pheromone init, candidate build or a pheromone upload). */
+/* score[slot] = taupow[tau_cand[slot]] * eta[d(slot)] for every slot, and
+ * the all-pairs weights when enabled (after pheromone init, candidate build
+ * or a pheromone upload). */
 __global__ void __launch_bounds__(256) k_refresh_scores(DevCtx P) {
   const int colony = blockIdx.y;
   const ColonyDesc D = P.col[colony];
@@ -572,6 +700,9 @@ __global__ void __launch_bounds__(256) k_refresh_scores(DevCtx P) {
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < slots;
        e += (size_t)gridDim.x * blockDim.x)
     score[e] = P.taupow[tc[e]] * P.eta[rec[e] >> 16];
+  if (P.fscore)
+    fs_items(P, D, I, P.taupow, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x,
+             false, 0u);
 }
 
 /* ---- best selection, stop rules and deposits (one workgroup per colony) --- */
@@ -708,8 +839,8 @@ __device__ __forceinline__ void patch_slot(const DevCtx &P, const UpdTables &T,
  * edge; the only entries two edges share are those of a single-customer
  * route 0-c-0 (one thread does both edges) and the (0,0) entry of empty
  * routes (one thread applies them all), so the result equals the sequential
- * edge order.  The candidate-aligned copy and its scores are patched
- * alongside.  Two dependent memory round trips per edge.  Ends with a
+ * edge order.  The candidate-aligned copy, its scores and the all-pairs
+ * weights are patched alongside.  Two dependent memory round trips per edge.  Ends with a
  * workgroup barrier. */
 __device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDesc &D,
                              const InstDesc &I, const int32_t *tour, int len, float delta) {
@@ -720,6 +851,8 @@ __device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDe
   float *__restrict__ score = P.score + D.score_off;
   const uint8_t *__restrict__ so = P.slot_of + I.slot_off;
   const int2 *__restrict__ xy = P.xy + I.node_off;
+  float *__restrict__ fs = P.fscore ? P.fscore + D.fs_off : nullptr;
+  const int S = I.fs_stride;
   if (threadIdx.x == 0) {
     s_zz_count = 0;
     s_zz_first = 0x7FFFFFFF;
@@ -745,12 +878,13 @@ __device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDe
         }
         tau[0] = (uint8_t)x;
         patch_slot(P, T, I, tc, score, 0, 0, so[0], 0, (uint8_t)x);
+        if (fs) fs[0] = T.taupow[x] * T.eta[0];
       }
       continue;
     }
     if (u != 0 && v == 0 && um1 == 0) continue; /* done by thread p-1 */
-    const uint32_t a = tau[(size_t)u * n1 + v];
-    const uint32_t b = tau[(size_t)v * n1 + u];
+    const uint32_t a = tau[(size_t)u * S + v];
+    const uint32_t b = tau[(size_t)v * S + u];
     const int su = so[(size_t)u * n1 + v];
     const int sv = so[(size_t)v * n1 + u];
     const int2 pu = xy[u], pv = xy[v];
@@ -769,10 +903,14 @@ __device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDe
       tuv = nq > a ? nq : a;
       tvu = nq > b ? nq : b;
     }
-    tau[(size_t)u * n1 + v] = (uint8_t)tuv;
-    tau[(size_t)v * n1 + u] = (uint8_t)tvu;
+    tau[(size_t)u * S + v] = (uint8_t)tuv;
+    tau[(size_t)v * S + u] = (uint8_t)tvu;
     patch_slot(P, T, I, tc, score, u, v, su, d, (uint8_t)tuv);
     patch_slot(P, T, I, tc, score, v, u, sv, d, (uint8_t)tvu);
+    if (fs) {
+      fs[(size_t)u * S + v] = T.taupow[tuv] * T.eta[d];
+      fs[(size_t)v * S + u] = T.taupow[tvu] * T.eta[d];
+    }
   }
   __syncthreads();
 }
@@ -829,17 +967,25 @@ __global__ void __launch_bounds__(1024) k_update(DevCtx P, int mode) {
  * every block's evaporation visible to that last block. */
 __global__ void __launch_bounds__(1024) k_evap_update(DevCtx P, int mat_blocks) {
   __shared__ int s_last;
+  __shared__ float s_tp[256];
   const int colony = blockIdx.y;
   AcoHipColonyState *S = P.state + colony;
   const ColonyDesc D = P.col[colony];
   const bool halted = S->halted != 0;
+  unsigned long long *stc = P.stats ? P.stats + (size_t)colony * ACO_STATS : nullptr;
+  if (stc && threadIdx.x == 0) atomicMax(stc + 10, ~wall_clock64());
   int any = 0;
   if (!halted)
     for (int a = threadIdx.x; a < D.m; a += blockDim.x) any |= P.unvis[D.ant_off + a] == 0;
   const bool run = !halted && __syncthreads_or(any);
   if (run) {
     const uint32_t d = P.q_evap;
-    if ((int)blockIdx.x < mat_blocks) {
+    if ((int)blockIdx.x < mat_blocks && P.fscore) {
+      for (int i = threadIdx.x; i < 256; i += blockDim.x) s_tp[i] = P.taupow[i];
+      __syncthreads();
+      fs_items(P, D, P.inst[D.inst], s_tp, blockIdx.x * blockDim.x + threadIdx.x,
+               (uint32_t)mat_blocks * blockDim.x, true, d);
+    } else if ((int)blockIdx.x < mat_blocks) {
       uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
       const size_t nvec = (size_t)D.mat_bytes / 16;
       for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nvec;
@@ -867,27 +1013,44 @@ __global__ void __launch_bounds__(1024) k_evap_update(DevCtx P, int mat_blocks) 
       }
     }
   }
-  __threadfence();
+  /* the barrier orders every wave's stores before thread 0's device-scope
+   * release; one fence per block (each fence writes back the L2) */
   __syncthreads();
   if (threadIdx.x == 0) {
+    __threadfence();
     const int prev = atomicAdd(P.done + colony, 1);
     s_last = prev == (int)gridDim.x - 1;
+    if (s_last) {
+      __threadfence(); /* acquire: the other blocks' evaporation */
+      P.done[colony] = 0; /* ready for the next epoch */
+    }
   }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
-  if (threadIdx.x == 0) P.done[colony] = 0; /* ready for the next epoch */
+  const unsigned long long t0 = stc ? wall_clock64() : 0ull;
+  if (stc && threadIdx.x == 0) {
+    const unsigned long long first = ~stc[10];
+    stc[10] = 0ull;
+    stc[7] += t0 - first;
+  }
   if (halted) {
     if (threadIdx.x == 0) S->found = 0;
     return;
   }
-  if (!select_best_block(P, colony)) return;
+  const bool found = select_best_block(P, colony);
+  const unsigned long long t1 = stc ? wall_clock64() : 0ull;
+  if (stc && threadIdx.x == 0) {
+    stc[6] += 1;
+    stc[8] += t1 - t0;
+  }
+  if (!found) return;
   const InstDesc I = P.inst[D.inst];
   const UpdTables T = stage_tables(P);
   const int ant = S->best_ant;
   deposit_tour(P, T, D, I, P.tours + D.tour_off + (size_t)ant * D.stride, P.len[D.ant_off + ant],
                S->dep_iter);
   deposit_tour(P, T, D, I, P.best + D.best_off, S->best_len, S->dep_best);
+  if (stc && threadIdx.x == 0) stc[9] += wall_clock64() - t1;
 }
 
 /* Deposit of a host-provided tour (staged in the scratch tour buffer). */

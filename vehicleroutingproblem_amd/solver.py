@@ -128,11 +128,13 @@ class Params:
     knn_key: int = N.KNN_EXACT
     ants_per_group: int = 0
     stage_lds: int = -1
+    fallback_matrix: int = -1
 
     def to_c(self) -> N.AcoHipParams:
         p = N.AcoHipParams()
         for f in ("cand_k", "ants", "colonies", "alpha", "beta", "rho", "tau0", "Q",
-                  "depot_close_weight", "knn_key", "ants_per_group", "stage_lds"):
+                  "depot_close_weight", "knn_key", "ants_per_group", "stage_lds",
+                  "fallback_matrix"):
             setattr(p, f, getattr(self, f))
         return p
 
@@ -279,6 +281,14 @@ class HipSolver:
         k = self.cand_k(colony // self.params.colonies)
         out = np.zeros((inst.n + 1, k), np.float32)
         self._check(self.lib.aco_hip_get_scores(self._ctx, colony, N.ptr(out, C.c_float)))
+        return out
+
+    def fallback_weights(self, colony: int = 0) -> np.ndarray:
+        """(n+1, n+1) all-pairs fallback weights (requires fallback_matrix on)."""
+        inst = self.instance_of(colony)
+        out = np.zeros((inst.n + 1, inst.n + 1), np.float32)
+        self._check(self.lib.aco_hip_get_fallback_weights(self._ctx, colony,
+                                                          N.ptr(out, C.c_float)))
         return out
 
     def set_tau(self, tau: np.ndarray, colony: int = 0):
