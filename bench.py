@@ -92,6 +92,12 @@ def reference_openmp_baseline(seconds_budget=8.0):
                       f"candidate width), {t:.2f} s solver time"}
 
 
+def div_chunk(epochs: int, cap: int) -> int:
+    """Largest divisor of `epochs` up to `cap`: graph replays of that many
+    epochs end exactly on the last epoch."""
+    return max([c for c in range(1, min(cap, max(epochs, 1)) + 1) if epochs % c == 0] or [1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +105,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ants-per-group", type=int, default=0)
-    ap.add_argument("--chunk", type=int, default=32, help="epochs per graph replay")
+    ap.add_argument("--chunk", type=int, default=64,
+                    help="max epochs per graph replay (the largest divisor of --steps up to it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -114,12 +121,16 @@ def main():
     s.prepare()
     s.sync()
     # warmup (also captures the epoch graph)
-    s.run(fixed_epochs=args.warmup, chunk_epochs=min(args.warmup, args.chunk) or 1,
-          use_graph=True)
+    if args.warmup > 0:
+        s.run(fixed_epochs=args.warmup, chunk_epochs=div_chunk(args.warmup, args.chunk),
+              use_graph=True)
     s.sync()
-    # timed region: exactly `steps` epochs of the solver loop
+    # timed region: exactly `steps` epochs of the solver loop, in graph
+    # replays of `chunk` epochs (a divisor of steps, so no replay runs past
+    # the last epoch)
+    chunk = div_chunk(args.steps, args.chunk)
     t0 = time.perf_counter()
-    r = s.run(fixed_epochs=args.warmup + args.steps, chunk_epochs=args.chunk, use_graph=True)
+    r = s.run(fixed_epochs=args.warmup + args.steps, chunk_epochs=chunk, use_graph=True)
     s.sync()
     dt = time.perf_counter() - t0
     st = s.colony_state()
@@ -127,20 +138,32 @@ def main():
     assert epochs == args.steps, (epochs, args.steps)
     value = ANTS * epochs / dt
 
-    # per-kernel durations: hipEvents on the context stream around each launch
-    # (eager launches; the same epochs continue the colony)
-    prof_epochs = min(args.steps, 50)
-    s.set_timing(True)
-    s.run(fixed_epochs=st["epoch"] + prof_epochs, chunk_epochs=8, use_graph=False)
-    s.sync()
-    t = s.timing()
-    s.set_timing(False)
-    assert t["construct_launches"] >= prof_epochs, t
-    # device counters over a few more epochs
-    s.set_stats(True)
-    s.run(fixed_epochs=s.colony_state()["epoch"] + 10, chunk_epochs=10, use_graph=True)
-    dev = s.stats()
-    s.set_stats(False)
+    # Per-kernel durations over the same epochs as the timed region: a second
+    # solver with the same instance and seed replays them bit for bit (the
+    # path is deterministic), with hipEvents on its stream around every
+    # launch (eager launches).  A third replay collects the device counters.
+    def replay(timing: bool, stats: bool):
+        r_ = HipSolver(inst, p, seeds=[1])
+        r_.prepare()
+        if args.warmup > 0:
+            r_.run(fixed_epochs=args.warmup, chunk_epochs=div_chunk(args.warmup, args.chunk),
+                   use_graph=True)
+        r_.sync()
+        if timing:
+            r_.set_timing(True)
+        if stats:
+            r_.set_stats(True)
+        r_.run(fixed_epochs=args.warmup + args.steps, chunk_epochs=chunk,
+               use_graph=not timing)
+        r_.sync()
+        assert r_.colony_state()["global_best_cost"] == st["global_best_cost"]
+        out_ = r_.timing() if timing else r_.stats()
+        r_.close()
+        return out_
+
+    t = replay(True, False)
+    assert t["construct_launches"] == args.steps, t
+    dev = replay(False, True)
     construct_ms = t["construct_ms"] / max(1, t["construct_launches"])
     evap_ms = t["evaporate_ms"] / max(1, t["evaporate_launches"])
     k = min(CAND_K, inst.n)

@@ -156,9 +156,10 @@ static int fs_stride_of(int n1) { return (n1 + 15) & ~15; }
 /* LDS bytes of one construct workgroup for an instance of n1 nodes (layout
  * of k_construct): staged candidate rows (+64 padding entries), depot scores
  * and coordinates; the computed-fallback tables (taupow, and eta unless
- * eta_len == 0) when the weight matrix is off; W state arrays of S u16. */
-static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on) {
-  size_t b = (size_t)W * (size_t)fs_stride_of(n1) * 2;
+ * eta_len == 0) when the weight matrix is off; per wave a state array of S
+ * u16 and a tour buffer of `stride` u16. */
+static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on, int stride) {
+  size_t b = (size_t)W * ((size_t)fs_stride_of(n1) * 2 + r16((size_t)stride * 2));
   if (!fs_on) b += 256 * 4 + r16((size_t)eta_len * 4);
   if (stage) {
     const size_t ent = (size_t)n1 * k + 64;
@@ -422,7 +423,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     need = 0;
     for (int i = 0; i < num; ++i)
       need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0,
-                                    W, stage, c->fs_on));
+                                    W, stage, c->fs_on, c->max_stride));
   };
   /* preference: staged rows with the eta table, staged rows without it, then
    * (unless staging was forced) global rows, then fewer ants per workgroup */

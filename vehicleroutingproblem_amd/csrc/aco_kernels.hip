@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_k
   if (lane == 0) {
     const int2 p0 = xy[0];
     const int d0 = aco_nint_dist(pi.x, pi.y, p0.x, p0.y);
-    P.dscore[I.node_off + i] = depot_w * P.eta[d0];
+    const float ds = depot_w * P.eta[d0];
+    P.dscore[I.node_off + i] = ds == ds ? ds : 0.0f; /* NaN scores never close a route */
   }
 }
 
@@ -411,6 +412,9 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
     }
   }
   uint16_t *s_state = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * Sn * 2);
+  off += (size_t)W * Sn * 2;
+  /* tour buffer of this wave: entry p of the tour (u16 node ids) */
+  uint16_t *s_tour = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * round16((size_t)D.stride * 2));
   __syncthreads();
   if (STATS && threadIdx.x == 0)
     atomicAdd(P.stats + (size_t)colony * ACO_STATS + 5, clock64() - t_kernel);
@@ -442,32 +446,15 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   uint32_t s = 0;
   float rf0 = 0.0f, rf1 = 0.0f; /* lane l: uniform draws of decision (s & ~63) + l */
   int pos = 0;
-  int tb = 0;      /* tour buffer: lane (p & 63) holds entry p of the current block */
-  int carry = 0;   /* last entry of the previous block */
-  long long cost = 0;
-
-  auto flush = [&](int base, int cnt) {
-    const int e = tb;
-    int prev = __shfl_up(e, 1, 64);
-    if (lane == 0) prev = carry;
-    int dd = 0;
-    if (lane < cnt) {
-      if (base + lane >= 1) {
-        const int2 a = c_xy[prev], b = c_xy[e];
-        dd = aco_nint_dist(a.x, a.y, b.x, b.y);
-      }
-      tour_out[base + lane] = e;
-    }
-    cost += wave_sum_i64(dd);
-    carry = __builtin_amdgcn_readlane(e, cnt - 1);
-  };
+  if (lane == 0) s_tour[0] = 0;
 
   for (int veh = 0; veh < K; ++veh) {
     int cur = 0;
     int load = 0;
     const int close_cap = (K - veh - 1) * Q; /* close allowed iff rem_dem <= close_cap */
-    for (;;) {
-      if (rem_cnt <= 0 || load >= Q) break; /* aco_cuda_kernels.cu:492 */
+    /* aco_cuda_kernels.cu:492: the route goes on while rem_cnt > 0 and
+     * load < Q; cur = -1 marks a route closed by a roulette (one exit test) */
+    while (((rem_cnt - 1) | (Q - 1 - load) | cur) >= 0) {
       const int room = Q - load;
       const int sl = (int)(s & 63u);
       if (sl == 0) {
@@ -512,33 +499,52 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
           ++fallbacks;
         }
       }
-      /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand */
-      if (move > 0 && rem_dem <= close_cap) {
-        const float dsc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dsc_cur)));
-        if (dsc > 0.0f) {
-          const float thr2 = f_readlane(rf1, sl) * (T + dsc);
+      /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand;
+       * depot scores are finite or 0 (never NaN), so dsc > 0 is a test of
+       * the sign-and-magnitude bits.  A fallback that found nothing (move
+       * -1) closes the route whatever this roulette draws. */
+      if (rem_dem <= close_cap) {
+        const int dbits = __builtin_amdgcn_readfirstlane(__float_as_int(dsc_cur));
+        if (dbits > 0) {
+          const float thr2 = f_readlane(rf1, sl) * (T + __int_as_float(dbits));
           if (thr2 >= T) move = 0;
         }
       }
-      if (move <= 0) break;
-      ++pos;
-      if (lane == (pos & 63)) tb = move;
-      if ((pos & 63) == 63) flush(pos - 63, 64);
-      if (lane == 0) s_state[move] = (uint16_t)VISITED;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      load += mdem;
-      rem_dem -= mdem;
-      --rem_cnt;
-      cur = move;
+      if (move > 0) {
+        ++pos;
+        if (lane == 0) {
+          s_state[move] = (uint16_t)VISITED;
+          s_tour[pos] = (uint16_t)move;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        load += mdem;
+        rem_dem -= mdem;
+        --rem_cnt;
+        cur = move;
+      } else {
+        cur = -1; /* nothing fits (fallback found none) or the depot closes */
+      }
     }
     /* route close (aco_cuda_kernels.cu:547-552) */
     ++pos;
-    if (lane == (pos & 63)) tb = 0;
-    if ((pos & 63) == 63) flush(pos - 63, 64);
+    if (lane == 0) s_tour[pos] = 0;
   }
-  if ((pos & 63) != 63) flush(pos & ~63, (pos & 63) + 1);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  /* write the tour out and sum its nint edge lengths (int64, exact) */
+  long long cost = 0;
+  for (int p = lane; p <= pos; p += 64) {
+    const int v = s_tour[p];
+    tour_out[p] = v;
+    if (p >= 1) {
+      const int2 a = c_xy[s_tour[p - 1]], b = c_xy[v];
+      cost += aco_nint_dist(a.x, a.y, b.x, b.y);
+    }
+  }
+  cost = wave_sum_i64(cost);
   if (lane == 0) {
     const int a = D.ant_off + ant;
     P.len[a] = pos + 1;
