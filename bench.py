@@ -107,7 +107,19 @@ def main():
     ap.add_argument("--ants-per-group", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=64,
                     help="max epochs per graph replay (the largest divisor of --steps up to it)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device twice (calibrates the HBM "
+                         "byte counters of a profiler against a copy of known size)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 2}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "sharding of this path is a design note, not a built path.", file=sys.stderr)

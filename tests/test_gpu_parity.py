@@ -35,6 +35,14 @@ CASES = {
     # full coordinate range: exercises the nint square root near 2*32767^2
     "wide_coords": (lambda: Instance.uniform(200, 4, side=32767, seed=5),
                     dict(cand_k=24, ants=32, rho=0.3)),
+    # customers beside (and on) the depot with a capacity of a third of the
+    # demand: best tours hold empty (0,0) routes and single-customer 0-c-0
+    # routes, the conflict cases of the deposits
+    "short_routes": (lambda: Instance(
+        np.array([500, 500, 500, 500, 438, 433, 858, 85, 697, 201, 94, 526, 975, 735, 761]),
+        np.array([500, 498, 497, 500, 128, 839, 450, 500, 370, 182, 926, 781, 643, 402, 822]),
+        np.array([0, 7, 1, 7, 6, 2, 5, 2, 6, 5, 3, 1, 7, 4, 7]), 4, 21, "short_routes"),
+        dict(cand_k=6, ants=64, rho=0.5)),
     # short candidate lists: most late decisions take the all-customer
     # fallback, over three 1024-node chunks
     "n2100_k6_fallbacks": (lambda: Instance.uniform(2100, 12, seed=21),
@@ -131,13 +139,20 @@ def test_construct_matches_oracle(gpu, name, stage, fm):
 
 @pytest.mark.parametrize("name,fm", [("c1_n100", 1), ("n300_clustered", 1), ("smoke_unit", 1),
                                      ("n1030_two_chunks", 1), ("n1030_two_chunks", 0),
-                                     ("n2100_k6_fallbacks", 1)])
+                                     ("n2100_k6_fallbacks", 1), ("short_routes", 1)])
 def test_epochs_match_oracle(gpu, name, fm):
     inst, s, o = make_pair(name, fallback_matrix=fm)
     s.prepare()
+    empty = single = 0
     for e in range(6):
         s.epochs(1)
         o.epochs(1)
+        gt, gl, _, _ = s.tours()
+        b = o.state()["best_ant"]
+        if b >= 0:
+            t = gt[b, :gl[b]]
+            empty += int(np.any((t[:-1] == 0) & (t[1:] == 0)))
+            single += int(np.any((t[:-2] == 0) & (t[2:] == 0) & (t[1:-1] != 0)))
         gs, os_ = s.colony_state(), o.state()
         for key in ("epoch", "iter_since_best", "found", "best_ant", "best_len",
                     "feasible_ants", "iter_best_cost", "global_best_cost", "improved"):
@@ -158,6 +173,8 @@ def test_epochs_match_oracle(gpu, name, fm):
         bt, bc = s.best()
         ot, oc = o.best()
         assert bc == oc and np.array_equal(bt, ot)
+    if name == "short_routes":
+        assert empty > 0 and single > 0, (empty, single)
     s.close()
 
 
