@@ -208,16 +208,17 @@ def main():
                      "evaporate_avg_ms": round(evap_ms, 5)},
         "kernel_ms_per_epoch": {
             "construct": round(t["construct_ms"] / max(1, t["construct_launches"]), 5),
-            "evaporate_and_update": round(t["evaporate_ms"] / max(1, t["construct_launches"]), 5),
-            "update": round((t["best_ms"] + t["deposit_ms"]) / max(1, t["construct_launches"]), 5)},
+            "evaporate": round(t["evaporate_ms"] / max(1, t["construct_launches"]), 5),
+            "select_and_deposit": round((t["best_ms"] + t["deposit_ms"])
+                                        / max(1, t["construct_launches"]), 5)},
         "construct_counters": {
             "decisions_per_tour": round(dev["decisions"] / max(1, dev["ants"]), 1),
             "fallback_rate": round(dev["fallbacks"] / max(1, dev["decisions"]), 4),
             "fallback_cycle_share": round(dev["fallback_cycles"] / max(1, dev["ant_cycles"]), 4),
             "cycles_per_decision": round(dev["ant_cycles"] / max(1, dev["decisions"]), 1)},
-        "update_phases_us": {
-            k: round(1e6 * dev[k + "_ticks"] / max(1, dev["wall_clock_hz"]) / max(1, dev["updates"]),
-                     2) for k in ("evap", "select", "deposit")},
+        "update_phases_us": ({
+            k: round(1e6 * dev[k + "_ticks"] / max(1, dev["wall_clock_hz"]) / dev["updates"], 2)
+            for k in ("evap", "select", "deposit")} if dev["updates"] else None),
     }
     s.close()
     if not args.no_cpu:

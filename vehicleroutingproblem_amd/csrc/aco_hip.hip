@@ -42,6 +42,9 @@ struct AcoHipContext {
   int fs_on = 0;       /* all-pairs fallback weight matrix allocated */
   size_t dep_lds = 0;  /* dynamic LDS of the deposit pair, 0 = sequential deposits */
   int dep_eta = 0;     /* the deposit pair stages the eta table in LDS */
+  size_t sd_lds = 0;   /* dynamic LDS of k_select_deposit, 0 = fused k_evap_update path */
+  int sd_eta = 0;      /* k_select_deposit stages the eta table in LDS */
+  int sd_blocks = 1;   /* workgroups of k_select_deposit per colony */
   size_t lds_bytes = 0;
   std::vector<InstDesc> inst;
   std::vector<ColonyDesc> col;
@@ -459,6 +462,14 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   /* the eta table joins the tours in LDS while it is small */
   c->dep_eta = c->dep_lds && c->eta_len <= 8192;
   if (c->dep_eta) c->dep_lds += (size_t)c->eta_len * 4;
+  /* epochs as evaporate + multi-workgroup select/deposit while both tours
+   * and their position tables fit in LDS */
+  size_t sd = 0;
+  for (int i = 0; i < num; ++i) sd = std::max(sd, sel_dep_lds(c->inst_n1[i], c->max_stride));
+  c->sd_eta = c->eta_len <= 8192;
+  if (c->sd_eta) sd += r16((size_t)c->eta_len * 4);
+  c->sd_lds = (sd <= 131072 && c->max_stride <= 65536) ? sd : 0;
+  c->sd_blocks = std::max(1, std::min(64, (2 * c->max_stride + 255) / 256));
   c->W = W;
   c->stage = stage;
   c->eta_lds = eta_in_lds;
@@ -562,6 +573,8 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
 #undef ACO_VARIANTS
   for (const void *f : variants)
     HIPC(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
+  HIPC(c, hipFuncSetAttribute((const void *)k_select_deposit,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
   c->loaded = true;
   return ACO_HIP_OK;
 }
@@ -692,6 +705,16 @@ static int launch_evap_update(AcoHipContext *c) {
   return ACO_HIP_OK;
 }
 
+static int launch_select_deposit(AcoHipContext *c) {
+  {
+    Timed tm(c, T_BEST);
+    hipLaunchKernelGGL(k_select_deposit, dim3(c->sd_blocks, c->num_col), dim3(256), c->sd_lds,
+                       c->stream, c->P, c->sd_eta);
+  }
+  LAUNCHED(c, "k_select_deposit");
+  return ACO_HIP_OK;
+}
+
 static int launch_refresh(AcoHipContext *c) {
   const int work = std::max(c->max_slots, c->fs_on ? c->max_fs_items : 0);
   const int xs = std::max(1, std::min((work + 255) / 256, 1024));
@@ -724,10 +747,17 @@ extern "C" int aco_hip_deposit_best(AcoHipContext *c) {
 static int launch_epochs(AcoHipContext *c, int count) {
   int rc;
   for (int e = 0; e < count; ++e) {
-    /* evaporation reads feasibility from the ants, so it runs before the
-     * fused best-selection + deposits workgroup: three launches per epoch */
+    /* construct, then evaporation (it reads feasibility from the ants),
+     * then best selection + both deposits over many workgroups; instances
+     * whose tours do not fit the LDS tables of that kernel use the fused
+     * evaporate + single-workgroup update kernel instead */
     if ((rc = launch_construct(c))) return rc;
-    if ((rc = launch_evap_update(c))) return rc;
+    if (c->sd_lds) {
+      if ((rc = launch_evap(c, 1))) return rc;
+      if ((rc = launch_select_deposit(c))) return rc;
+    } else {
+      if ((rc = launch_evap_update(c))) return rc;
+    }
   }
   return ACO_HIP_OK;
 }

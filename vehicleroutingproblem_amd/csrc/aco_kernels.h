@@ -142,6 +142,13 @@ __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
 __global__ void k_refresh_scores(DevCtx P);
 __global__ void k_update(DevCtx P, int mode);
 __global__ void k_evap_update(DevCtx P, int mat_blocks, int dep_fast);
+__global__ void k_select_deposit(DevCtx P, int stage_eta);
+
+/* LDS bytes of k_select_deposit: both tours (int32) and the position of
+ * every customer in each (u16). */
+static inline __host__ __device__ size_t sel_dep_lds(int n1, int stride) {
+  return (size_t)stride * 8 + (size_t)((n1 + 7) & ~7) * 4;
+}
 __global__ void k_build_slots(DevCtx P, int total_rows);
 __global__ void k_deposit_tour(DevCtx P, int colony, const int32_t *tour, int len, float delta);
 __global__ void k_route_costs(DevCtx P, int instance, const int32_t *tours, const int32_t *lens,

@@ -1088,6 +1088,251 @@ __device__ void deposit_pair(const DevCtx &P, const UpdTables &T0, const ColonyD
   }
 }
 
+/* ---- best selection and both deposits, spread over many workgroups -------- */
+
+/* Events of tour t on the unordered pheromone pair {x, y} (not both 0):
+ * returns their count (0..2) and each event's forward node (the event
+ * "from -> to" is an edge of the tour, in tour order).  Every customer
+ * appears once in a complete tour, so the pair is an edge iff x and y are
+ * adjacent; a depot-customer pair can be both edges of a route 0-c-0. */
+__device__ __forceinline__ int pair_events(const int32_t *t, int len, const uint16_t *pos, int x,
+                                           int y, int (&from)[2]) {
+  int n = 0;
+  if (x != 0 && y != 0) {
+    const int p = pos[x];
+    if (p + 1 < len && t[p + 1] == y) from[n++] = x;
+    else if (p >= 1 && t[p - 1] == y) from[n++] = y;
+    return n;
+  }
+  const int c = x ? x : y;
+  const int p = pos[c];
+  if (p >= 1 && t[p - 1] == 0) from[n++] = 0;
+  if (p + 1 < len && t[p + 1] == 0) from[n++] = c;
+  return n;
+}
+
+/* Applies one deposit event to the pair (A = tau[x][y], B = tau[y][x]):
+ * the forward entry's new level max-es into both. */
+__device__ __forceinline__ void pair_apply(const UpdTables &T, bool fwd_x, float delta, uint32_t &A,
+                                           uint32_t &B) {
+  const uint32_t nq = dep_new(T, fwd_x ? A : B, delta);
+  A = nq > A ? nq : A;
+  B = nq > B ? nq : B;
+}
+
+/* One epoch's best selection and deposits after k_evaporate(from_ants = 1).
+ * Every workgroup repeats the (cheap, deterministic) iteration-best
+ * selection, stages both tours and the position of every customer in each
+ * (LDS), and owns a share of the unordered pheromone pairs {u, v} the two
+ * tours touch: for its pair it applies tour 1's events (iteration best,
+ * delta1) then tour 2's (global best, delta2) - per pair exactly the result
+ * of the sequential deposits of kernel_deposit_solution, since pairs share
+ * no entries.  The empty-route (0,0) entry is one thread's.  The last
+ * workgroup to finish (device counter) commits the colony state, the
+ * global-best tour and the stop rules (aco_cuda.cu:104-126,331-399). */
+__global__ void __launch_bounds__(256) k_select_deposit(DevCtx P, int stage_eta) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char dyn_smem[];
+  __shared__ unsigned long long s_min[4];
+  __shared__ int s_cnt[4], s_last, s_zz[2];
+  __shared__ float s_deq[256], s_thr[256], s_tp[256];
+  const int colony = blockIdx.y;
+  AcoHipColonyState *S = P.state + colony;
+  const ColonyDesc D = P.col[colony];
+  const InstDesc I = P.inst[D.inst];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nt = (int)blockDim.x;
+  const bool halted = S->halted != 0;
+  const long long gbest = S->global_best_cost;
+  const int best_len0 = S->best_len;
+
+  /* iteration best: feasible ants only, lowest cost, ties to the lower ant */
+  unsigned long long key = ~0ull;
+  int cnt = 0;
+  if (!halted) {
+    for (int a = threadIdx.x; a < D.m; a += nt) {
+      if (P.unvis[D.ant_off + a] == 0) {
+        const unsigned long long kk =
+            ((unsigned long long)P.cost[D.ant_off + a] << 20) | (unsigned long long)a;
+        key = kk < key ? kk : key;
+        ++cnt;
+      }
+    }
+  }
+  key = wave_min_u64(key);
+  cnt = (int)wave_sum_i64(cnt);
+  if (lane == 0) {
+    s_min[wave] = key;
+    s_cnt[wave] = cnt;
+  }
+  for (int i = threadIdx.x; i < 256; i += nt) {
+    s_deq[i] = P.deq[i];
+    s_thr[i] = i < 255 ? P.thr[i] : 0.0f;
+    s_tp[i] = P.taupow[i];
+  }
+  if (threadIdx.x < 2) s_zz[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long best = s_min[0];
+  int feasible = s_cnt[0];
+  for (int w = 1; w < (nt >> 6); ++w) {
+    best = s_min[w] < best ? s_min[w] : best;
+    feasible += s_cnt[w];
+  }
+  const bool found = best != ~0ull;
+  const int ant = found ? (int)(best & 0xFFFFFu) : -1;
+  const long long c = found ? (long long)(best >> 20) : INT64_MAX;
+  const bool improved = found && c < gbest;
+  const long long gnew = improved ? c : gbest;
+  const float qeff = P.qdep * 100.0f;
+  const float delta1 = found ? (0.3f * qeff) / (float)c : 0.0f;
+  const float delta2 = found ? (0.7f * qeff) / (float)gnew : 0.0f;
+  const int32_t *g1 = found ? P.tours + D.tour_off + (size_t)ant * D.stride : nullptr;
+  const int len1 = found ? P.len[D.ant_off + ant] : 0;
+  const int32_t *g2 = improved ? g1 : P.best + D.best_off;
+  const int len2 = improved ? len1 : best_len0;
+
+  if (found) {
+    const int n1 = I.n + 1, Sn = I.fs_stride, stride = D.stride;
+    int32_t *t1 = reinterpret_cast<int32_t *>(dyn_smem);
+    int32_t *t2 = t1 + stride;
+    uint16_t *pos1 = reinterpret_cast<uint16_t *>(t2 + stride);
+    uint16_t *pos2 = pos1 + ((n1 + 7) & ~7);
+    UpdTables T;
+    T.deq = s_deq;
+    T.thr = s_thr;
+    T.taupow = s_tp;
+    T.eta = P.eta;
+    if (stage_eta) {
+      float *s_eta = reinterpret_cast<float *>(pos2 + ((n1 + 7) & ~7));
+      for (int i = threadIdx.x; i < P.eta_len; i += nt) s_eta[i] = P.eta[i];
+      T.eta = s_eta;
+    }
+    for (int p = threadIdx.x; p < len1; p += nt) {
+      const int v = g1[p];
+      t1[p] = v;
+      if (v != 0) pos1[v] = (uint16_t)p;
+    }
+    for (int p = threadIdx.x; p < len2; p += nt) {
+      const int v = g2[p];
+      t2[p] = v;
+      if (v != 0) pos2[v] = (uint16_t)p;
+    }
+    __syncthreads();
+    uint8_t *__restrict__ tau = P.pher + D.pher_off;
+    uint8_t *__restrict__ tc = P.pher + D.tcand_off;
+    float *__restrict__ score = P.score + D.score_off;
+    float *__restrict__ fs = P.fscore ? P.fscore + D.fs_off : nullptr;
+    const uint8_t *__restrict__ so = P.slot_of + I.slot_off;
+    const int2 *__restrict__ xy = P.xy + I.node_off;
+    const int e1 = len1 - 1, e2 = len2 - 1;
+    /* the (0,0) pair: empty routes of either tour, one thread */
+    if (blockIdx.x == 0) {
+      for (int p = threadIdx.x; p < e1; p += nt)
+        if (t1[p] == 0 && t1[p + 1] == 0) atomicAdd(&s_zz[0], 1);
+      for (int p = threadIdx.x; p < e2; p += nt)
+        if (t2[p] == 0 && t2[p + 1] == 0) atomicAdd(&s_zz[1], 1);
+      __syncthreads();
+      if (threadIdx.x == 0 && (s_zz[0] | s_zz[1])) {
+        uint32_t x = tau[0];
+        for (int t = 0; t < 2; ++t)
+          for (int r = 0; r < s_zz[t]; ++r) {
+            const uint32_t nq = dep_new(T, x, t ? delta2 : delta1);
+            x = nq > x ? nq : x;
+          }
+        tau[0] = (uint8_t)x;
+        patch_slot(P, T, I, tc, score, 0, 0, so[0], 0, (uint8_t)x);
+        if (fs) fs[0] = T.taupow[x] * T.eta[0];
+      }
+    }
+    for (int it = blockIdx.x * nt + threadIdx.x; it < e1 + e2; it += gridDim.x * nt) {
+      const bool second = it >= e1;
+      const int p = second ? it - e1 : it;
+      const int32_t *tt = second ? t2 : t1;
+      const int x = tt[p], y = tt[p + 1];
+      if (x == 0 && y == 0) continue;
+      if (y == 0 && p >= 1 && tt[p - 1] == 0) continue; /* pair {0,x} done at p-1 */
+      int f1[2], f2[2];
+      const int n1e = pair_events(t1, len1, pos1, x, y, f1);
+      if (second && n1e) continue; /* the tour-1 item of this pair applies both */
+      const int n2e = pair_events(t2, len2, pos2, x, y, f2);
+      const uint32_t xy_ = (uint32_t)x * (uint32_t)Sn + (uint32_t)y;
+      const uint32_t yx_ = (uint32_t)y * (uint32_t)Sn + (uint32_t)x;
+      uint32_t A = tau[xy_], B = tau[yx_];
+      const int sxy = so[(uint32_t)x * (uint32_t)n1 + (uint32_t)y];
+      const int syx = so[(uint32_t)y * (uint32_t)n1 + (uint32_t)x];
+      const int2 px = xy[x], py = xy[y];
+      for (int e = 0; e < n1e; ++e) pair_apply(T, f1[e] == x, delta1, A, B);
+      for (int e = 0; e < n2e; ++e) pair_apply(T, f2[e] == x, delta2, A, B);
+      const int d = aco_nint_dist(px.x, px.y, py.x, py.y);
+      tau[xy_] = (uint8_t)A;
+      tau[yx_] = (uint8_t)B;
+      patch_slot(P, T, I, tc, score, x, y, sxy, d, (uint8_t)A);
+      patch_slot(P, T, I, tc, score, y, x, syx, d, (uint8_t)B);
+      if (fs) {
+        fs[xy_] = T.taupow[A] * T.eta[d];
+        fs[yx_] = T.taupow[B] * T.eta[d];
+      }
+    }
+  }
+
+  /* the last workgroup commits the colony state */
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int prev = atomicAdd(P.done + colony, 1);
+    s_last = prev == (int)gridDim.x - 1;
+    if (s_last) {
+      __threadfence();
+      P.done[colony] = 0;
+    }
+  }
+  __syncthreads();
+  if (!s_last) return;
+  if (halted) {
+    if (threadIdx.x == 0) S->found = 0;
+    return;
+  }
+  if (threadIdx.x == 0) {
+    S->feasible_ants = feasible;
+    S->improved = improved ? 1 : 0;
+    if (found) {
+      S->found = 1;
+      S->best_ant = ant;
+      S->iter_best_cost = c;
+      if (improved) {
+        const double prev =
+            gbest == INT64_MAX ? 1.7976931348623157e308 : (double)gbest;
+        if (aco_significant(prev, (double)c, P.min_rel)) S->iter_since_best = 0;
+        else S->iter_since_best++;
+        S->global_best_cost = c;
+        S->best_len = len1;
+      } else {
+        S->iter_since_best++;
+      }
+      S->dep_iter = delta1;
+      S->dep_best = delta2;
+    } else {
+      S->found = 0;
+      S->best_ant = -1;
+      S->iter_best_cost = INT64_MAX;
+      S->iter_since_best++;
+    }
+    S->epoch++;
+    if (P.fixed_epochs > 0) S->halted = S->epoch >= P.fixed_epochs;
+    else if (P.stagnation > 0 && S->iter_since_best >= P.stagnation) S->halted = 1;
+    if (P.max_epochs > 0 && S->epoch >= P.max_epochs) S->halted = 1;
+    HistEntry *h = reinterpret_cast<HistEntry *>(P.hist) + (size_t)colony * ACO_HIST +
+                   (S->epoch - 1) % ACO_HIST;
+    h->epoch = S->epoch;
+    h->iter_since_best = S->iter_since_best;
+    h->improved = S->improved;
+    h->halted = S->halted;
+    h->global_best = S->global_best_cost;
+  }
+  if (improved) {
+    int32_t *dst = P.best + D.best_off;
+    for (int p = threadIdx.x; p < D.stride; p += nt) dst[p] = g1[p];
+  }
+}
+
 /* mode bit 1: best selection and stop rules; bit 2: iteration-best deposit;
  * bit 4: global-best deposit.  Deposits run only when the last best
  * selection found a feasible ant (aco_cuda.cu:355-396).  Global writes of one
