@@ -22,6 +22,11 @@ sys.path.insert(0, ROOT)
 
 N_CUST, K_VEH, ANTS, CAND_K, RHO = 1000, 8, 256, 20, 0.1
 HBM_PEAK_GBS = 8000.0
+# HBM bytes one k_construct launch of this workload moves, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes over this bench, each scaled by what the
+# counter reads for a 256 MiB device copy (fetch x2.0, write x1.0); the
+# commands and figures are in profiles/k_construct_n1000_m256_hbm_traffic.md.
+CONSTRUCT_TRAFFIC_BYTES = 20877046
 
 
 def tour_bytes(n, K, k):
@@ -202,7 +207,7 @@ def main():
                    "best_cost": st["global_best_cost"]},
         "roofline": {"bound": "hbm", "kernel": "k_construct", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": CONSTRUCT_TRAFFIC_BYTES,
                      "avg_launch_ms": round(construct_ms, 5),
                      "bytes_per_launch": bytes_per_launch,
                      "evaporate_avg_ms": round(evap_ms, 5)},
