@@ -178,6 +178,25 @@ def test_epochs_match_oracle(gpu, name, fm):
     s.close()
 
 
+@pytest.mark.parametrize("name", ["short_routes", "c1_n100"])
+def test_fused_update_matches_oracle(gpu, name, monkeypatch):
+    """The fused evaporate + single-workgroup update kernel (the path of
+    instances too large for the LDS tables of the multi-workgroup deposit)
+    gives the oracle's pheromone and state too."""
+    monkeypatch.setenv("ACO_HIP_FUSED_UPDATE", "1")
+    inst, s, o = make_pair(name)
+    s.prepare()
+    for e in range(5):
+        s.epochs(1)
+        o.epochs(1)
+        gs, os_ = s.colony_state(), o.state()
+        for key in ("epoch", "iter_since_best", "best_ant", "global_best_cost", "best_len"):
+            assert gs[key] == os_[key], (e, key)
+        assert np.array_equal(s.tau()[0], o.tau()[0]), f"tau differs after epoch {e}"
+        check_fallback_weights(s, o, inst)
+    s.close()
+
+
 def test_ants_per_group_invariance(gpu):
     ref = None
     for apg in (1, 3, 8):

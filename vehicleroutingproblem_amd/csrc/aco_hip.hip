@@ -40,8 +40,6 @@ struct AcoHipContext {
   int num_inst = 0, num_col = 0, W = 4, stage = 1, groups = 1, total_rows = 0, eta_lds = 1;
   int rows = 1;        /* 16-lane rows of the widest candidate list (k_construct ROWS) */
   int fs_on = 0;       /* all-pairs fallback weight matrix allocated */
-  size_t dep_lds = 0;  /* dynamic LDS of the deposit pair, 0 = sequential deposits */
-  int dep_eta = 0;     /* the deposit pair stages the eta table in LDS */
   size_t sd_lds = 0;   /* dynamic LDS of k_select_deposit, 0 = fused k_evap_update path */
   int sd_eta = 0;      /* k_select_deposit stages the eta table in LDS */
   int sd_blocks = 1;   /* workgroups of k_select_deposit per colony */
@@ -454,14 +452,6 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
                    "instance too large for the LDS layout (%zu bytes needed, stage=%d)", need,
                    stage);
-  /* one-pass deposits (both tours staged in LDS) while they fit */
-  size_t dep = 0;
-  for (int i = 0; i < num; ++i)
-    dep = std::max(dep, dep_pair_lds(c->inst_n1[i], c->max_stride));
-  c->dep_lds = (dep <= 49152 && c->max_stride <= 65536) ? dep : 0;
-  /* the eta table joins the tours in LDS while it is small */
-  c->dep_eta = c->dep_lds && c->eta_len <= 8192;
-  if (c->dep_eta) c->dep_lds += (size_t)c->eta_len * 4;
   /* epochs as evaporate + multi-workgroup select/deposit while both tours
    * and their position tables fit in LDS */
   size_t sd = 0;
@@ -469,6 +459,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   c->sd_eta = c->eta_len <= 8192;
   if (c->sd_eta) sd += r16((size_t)c->eta_len * 4);
   c->sd_lds = (sd <= 131072 && c->max_stride <= 65536) ? sd : 0;
+  if (env_int("ACO_HIP_FUSED_UPDATE", 0)) c->sd_lds = 0; /* force the fused update kernel */
   c->sd_blocks = std::max(1, std::min(64, (2 * c->max_stride + 255) / 256));
   c->W = W;
   c->stage = stage;
@@ -698,8 +689,8 @@ static int launch_evap_update(AcoHipContext *c) {
   const int xs = std::max(1, std::min((c->max_slots + 1023) / 1024, 256));
   {
     Timed tm(c, T_EVAP);
-    hipLaunchKernelGGL(k_evap_update, dim3(xm + xs, c->num_col), dim3(1024), c->dep_lds,
-                       c->stream, c->P, xm, c->dep_lds > 0 ? (c->dep_eta ? 2 : 1) : 0);
+    hipLaunchKernelGGL(k_evap_update, dim3(xm + xs, c->num_col), dim3(1024), 0, c->stream, c->P,
+                       xm);
   }
   LAUNCHED(c, "k_evap_update");
   return ACO_HIP_OK;

@@ -126,13 +126,6 @@ static inline __host__ __device__ int aco_significant(double prev, double now, d
   return rel + eps >= min_rel;
 }
 
-/* LDS bytes of the one-pass deposit of two tours (deposit_pair): both tours
- * (int32), the position of every customer in the first (u16) and the first
- * tour's per-edge results (2 x u8). */
-static inline __host__ __device__ size_t dep_pair_lds(int n1, int stride) {
-  return (size_t)stride * 8 + (size_t)((n1 + 7) & ~7) * 2 + (size_t)stride * 2;
-}
-
 /* kernels (aco_kernels.hip) */
 __global__ void k_knn(DevCtx P, int total_rows, int knn_key, float depot_w);
 __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
@@ -141,7 +134,7 @@ __global__ void k_construct(DevCtx P, int groups_per_colony, int W);
 __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
 __global__ void k_refresh_scores(DevCtx P);
 __global__ void k_update(DevCtx P, int mode);
-__global__ void k_evap_update(DevCtx P, int mat_blocks, int dep_fast);
+__global__ void k_evap_update(DevCtx P, int mat_blocks);
 __global__ void k_select_deposit(DevCtx P, int stage_eta);
 
 /* LDS bytes of k_select_deposit: both tours (int32) and the position of

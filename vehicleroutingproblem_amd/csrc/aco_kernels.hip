@@ -937,157 +937,6 @@ __device__ __forceinline__ UpdTables stage_tables(const DevCtx &P) {
   return T;
 }
 
-/* ---- both deposits of an epoch (update workgroup) ------------------------- */
-
-/* One edge's deposit in the sequential order of deposit_tour: new =
- * quantize(dequantize(tau[u][v]) + delta), max into tau[u][v] and tau[v][u]
- * (the reverse takes the forward's new value); a single-customer route 0-c-0
- * applies its second edge (c,0) to the results of the first. */
-__device__ __forceinline__ void dep_edge(const UpdTables &T, uint32_t a, uint32_t b, bool single,
-                                         float delta, uint32_t &tuv, uint32_t &tvu) {
-  uint32_t nq = dep_new(T, a, delta);
-  tuv = nq > a ? nq : a;
-  tvu = nq > b ? nq : b;
-  if (single) {
-    nq = dep_new(T, tvu, delta);
-    tvu = nq > tvu ? nq : tvu;
-    tuv = nq > tuv ? nq : tuv;
-  }
-}
-
-/* Value of pheromone entry (x, y) after the first tour's deposit, or -1 if
- * that deposit did not touch it: every customer appears once in a tour, so
- * an entry is touched iff x and y are adjacent there; pos[c] is customer c's
- * position, rf/rr the forward/reverse results of each edge.  (0,0) entries
- * are resolved by the caller. */
-__device__ __forceinline__ int dep_lookup(const int32_t *t1, int len1, const uint16_t *pos,
-                                          const uint8_t *rf, const uint8_t *rr, int x, int y) {
-  if (x != 0) {
-    const int p = pos[x];
-    if (p + 1 < len1 && t1[p + 1] == y) return rf[p];
-    if (p >= 1 && t1[p - 1] == y) return rr[p - 1];
-    return -1;
-  }
-  const int p = pos[y];
-  if (p >= 1 && t1[p - 1] == 0) return rf[p - 1];
-  if (p + 1 < len1 && t1[p + 1] == 0) return rr[p];
-  return -1;
-}
-
-/* Iteration-best deposit (tour 1, delta1) then global-best deposit (tour 2,
- * delta2), with the result of two sequential deposit_tour calls but fewer
- * memory round trips: both tours are staged in LDS once, and the second
- * deposit takes the entries the first one changed from LDS (dep_lookup)
- * instead of reading them back, so each deposit is one round of loads.  Both
- * tours must be complete tours (every customer once), which the iteration-
- * and global-best tours are.  `dyn` holds dep_pair_lds(n1, stride) bytes,
- * then (stage_eta) the eta table, staged alongside the tours.  Ends with a
- * workgroup barrier. */
-__device__ void deposit_pair(const DevCtx &P, const UpdTables &T0, const ColonyDesc &D,
-                             const InstDesc &I, const int32_t *g1, int len1, float delta1,
-                             const int32_t *g2, int len2, float delta2, unsigned char *dyn,
-                             bool stage_eta) {
-  __shared__ int s_zz_count[2], s_zz_first[2], s_zz_val;
-  const int n1 = I.n + 1, S = I.fs_stride;
-  const int stride = D.stride;
-  int32_t *t1 = reinterpret_cast<int32_t *>(dyn);
-  int32_t *t2 = t1 + stride;
-  uint16_t *pos = reinterpret_cast<uint16_t *>(t2 + stride);
-  uint8_t *rf = reinterpret_cast<uint8_t *>(pos + ((n1 + 7) & ~7));
-  uint8_t *rr = rf + stride;
-  UpdTables T = T0;
-  if (stage_eta) {
-    float *s_eta = reinterpret_cast<float *>(dyn + dep_pair_lds(n1, stride));
-    for (int i = threadIdx.x; i < P.eta_len; i += blockDim.x) s_eta[i] = P.eta[i];
-    T.eta = s_eta;
-  }
-  uint8_t *__restrict__ tau = P.pher + D.pher_off;
-  uint8_t *__restrict__ tc = P.pher + D.tcand_off;
-  float *__restrict__ score = P.score + D.score_off;
-  float *__restrict__ fs = P.fscore ? P.fscore + D.fs_off : nullptr;
-  const uint8_t *__restrict__ so = P.slot_of + I.slot_off;
-  const int2 *__restrict__ xy = P.xy + I.node_off;
-  const int nt = (int)blockDim.x;
-  if (threadIdx.x < 2) {
-    s_zz_count[threadIdx.x] = 0;
-    s_zz_first[threadIdx.x] = 0x7FFFFFFF;
-  }
-  for (int p = threadIdx.x; p < len1; p += nt) {
-    const int v = g1[p];
-    t1[p] = v;
-    if (v != 0) pos[v] = (uint16_t)p;
-  }
-  for (int p = threadIdx.x; p < len2; p += nt) t2[p] = g2[p];
-  __syncthreads();
-  for (int t = 0; t < 2; ++t) {
-    const int32_t *tt = t ? t2 : t1;
-    const int len = t ? len2 : len1;
-    for (int p = threadIdx.x; p + 1 < len; p += nt) {
-      if (tt[p] == 0 && tt[p + 1] == 0) {
-        atomicAdd(&s_zz_count[t], 1);
-        atomicMin(&s_zz_first[t], p);
-      }
-    }
-  }
-  __syncthreads();
-  for (int t = 0; t < 2; ++t) {
-    const int32_t *tt = t ? t2 : t1;
-    const int len = t ? len2 : len1;
-    const float delta = t ? delta2 : delta1;
-    for (int p = threadIdx.x; p + 1 < len; p += nt) {
-      const int u = tt[p], v = tt[p + 1];
-      if (u == 0 && v == 0) {
-        if (p != s_zz_first[t]) continue;
-        uint32_t x = (t == 1 && s_zz_count[0] > 0) ? (uint32_t)s_zz_val : (uint32_t)tau[0];
-        for (int r = 0; r < s_zz_count[t]; ++r) {
-          const uint32_t nq = dep_new(T, x, delta);
-          x = nq > x ? nq : x;
-        }
-        tau[0] = (uint8_t)x;
-        patch_slot(P, T, I, tc, score, 0, 0, so[0], 0, (uint8_t)x);
-        if (fs) fs[0] = T.taupow[x] * T.eta[0];
-        if (t == 0) s_zz_val = (int)x;
-        continue;
-      }
-      const int um1 = p >= 1 ? tt[p - 1] : -1;
-      if (u != 0 && v == 0 && um1 == 0) continue; /* second edge of a 0-c-0 route */
-      const bool single = u == 0 && p + 2 < len && tt[p + 2] == 0;
-      const uint32_t uv = (uint32_t)u * (uint32_t)S + (uint32_t)v;
-      const uint32_t vu = (uint32_t)v * (uint32_t)S + (uint32_t)u;
-      uint32_t a = tau[uv], b = tau[vu];
-      const int su = so[(uint32_t)u * (uint32_t)n1 + (uint32_t)v];
-      const int sv = so[(uint32_t)v * (uint32_t)n1 + (uint32_t)u];
-      const int2 pu = xy[u], pv = xy[v];
-      if (t == 1) {
-        const int la = dep_lookup(t1, len1, pos, rf, rr, u, v);
-        const int lb = dep_lookup(t1, len1, pos, rf, rr, v, u);
-        if (la >= 0) a = (uint32_t)la;
-        if (lb >= 0) b = (uint32_t)lb;
-      }
-      uint32_t tuv, tvu;
-      dep_edge(T, a, b, single, delta, tuv, tvu);
-      if (t == 0) {
-        rf[p] = (uint8_t)tuv;
-        rr[p] = (uint8_t)tvu;
-        if (single) { /* the route's second edge (c,0) ends with the same pair */
-          rf[p + 1] = (uint8_t)tvu;
-          rr[p + 1] = (uint8_t)tuv;
-        }
-      }
-      const int d = aco_nint_dist(pu.x, pu.y, pv.x, pv.y);
-      tau[uv] = (uint8_t)tuv;
-      tau[vu] = (uint8_t)tvu;
-      patch_slot(P, T, I, tc, score, u, v, su, d, (uint8_t)tuv);
-      patch_slot(P, T, I, tc, score, v, u, sv, d, (uint8_t)tvu);
-      if (fs) {
-        fs[uv] = T.taupow[tuv] * T.eta[d];
-        fs[vu] = T.taupow[tvu] * T.eta[d];
-      }
-    }
-    __syncthreads();
-  }
-}
-
 /* ---- best selection and both deposits, spread over many workgroups -------- */
 
 /* Events of tour t on the unordered pheromone pair {x, y} (not both 0):
@@ -1367,8 +1216,7 @@ __global__ void __launch_bounds__(1024) k_update(DevCtx P, int mode) {
  * waiting - runs the best selection and both deposits (k_update mode 7).
  * __threadfence() before the counter increment and after reading it makes
  * every block's evaporation visible to that last block. */
-__global__ void __launch_bounds__(1024) k_evap_update(DevCtx P, int mat_blocks, int dep_fast) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char dyn_smem[];
+__global__ void __launch_bounds__(1024) k_evap_update(DevCtx P, int mat_blocks) {
   __shared__ int s_last;
   __shared__ float s_tp[256];
   const int colony = blockIdx.y;
@@ -1450,15 +1298,9 @@ __global__ void __launch_bounds__(1024) k_evap_update(DevCtx P, int mat_blocks, 
   const InstDesc I = P.inst[D.inst];
   const UpdTables T = stage_tables(P);
   const int ant = S->best_ant;
-  const int32_t *g1 = P.tours + D.tour_off + (size_t)ant * D.stride;
-  const int len1 = P.len[D.ant_off + ant], len2 = S->best_len;
-  if (dep_fast) {
-    deposit_pair(P, T, D, I, g1, len1, S->dep_iter, P.best + D.best_off, len2, S->dep_best,
-                 dyn_smem, dep_fast == 2);
-  } else {
-    deposit_tour(P, T, D, I, g1, len1, S->dep_iter);
-    deposit_tour(P, T, D, I, P.best + D.best_off, len2, S->dep_best);
-  }
+  deposit_tour(P, T, D, I, P.tours + D.tour_off + (size_t)ant * D.stride, P.len[D.ant_off + ant],
+               S->dep_iter);
+  deposit_tour(P, T, D, I, P.best + D.best_off, S->best_len, S->dep_best);
   if (stc && threadIdx.x == 0) stc[9] += wall_clock64() - t1;
 }
 
