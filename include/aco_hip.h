@@ -190,7 +190,12 @@ int aco_hip_run(AcoHipContext *ctx, const AcoHipRunConfig *cfg, AcoHipRunStats *
 /* (n+1)*k candidate indices, nint distances and eta^beta values (any may be NULL). */
 int aco_hip_get_candidates(AcoHipContext *ctx, int instance, int32_t *idx, int32_t *dist,
                            float *eta);
-/* Replace the per-candidate heuristic (learned prior); eta has (n+1)*k values > 0. */
+/* Learned heuristic prior (the seam of aco_cuda_kernels.cu:295, where eta_beta
+ * enters the candidate weight): candidate slot (i, t) of the instance weighs
+ * taupow[tau_cand] * eta[i*k + t] instead of taupow * (1/d)^beta.  eta has
+ * (n+1)*k finite values >= 0 (row-major like aco_hip_get_candidates); NULL
+ * restores (1/d)^beta.  The all-customer fallback keeps (1/d)^beta.  Scores of
+ * every colony of the instance are refreshed; call before the next epoch. */
 int aco_hip_set_candidate_eta(AcoHipContext *ctx, int instance, const float *eta);
 /* (n+1)^2 pheromone bytes and (n+1)*k candidate-aligned copy (either may be NULL). */
 int aco_hip_get_tau(AcoHipContext *ctx, int colony, uint8_t *tau, uint8_t *tau_cand);

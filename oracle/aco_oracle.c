@@ -52,6 +52,7 @@ struct OracleColony {
   int dmax;
   /* candidates */
   uint32_t *rec;  /* idx | d << 16, (n+1)*k */
+  float *ceta;    /* learned heuristic per candidate slot, NULL = (1/d)^beta */
   float *dscore;  /* depot close score per node */
   /* pheromone */
   uint8_t *tau;      /* (n+1)^2 */
@@ -269,8 +270,20 @@ void oracle_destroy(OracleColony *o) {
   if (!o) return;
   free(o->x); free(o->y); free(o->dem); free(o->eta_lut); free(o->rec); free(o->dscore);
   free(o->tau); free(o->tau_cand); free(o->tours); free(o->len); free(o->unvisited);
-  free(o->cost); free(o->fallbacks); free(o->best_tour);
+  free(o->cost); free(o->fallbacks); free(o->best_tour); free(o->ceta);
   free(o);
+}
+
+/* Learned heuristic prior (the seam of aco_cuda_kernels.cu:295): candidate
+ * slot (i, t) weighs taupow[tau_cand] * eta[i*k + t] instead of
+ * taupow * (1/d)^beta.  The fallback keeps (1/d)^beta.  NULL restores it. */
+void oracle_set_candidate_eta(OracleColony *o, const float *eta) {
+  free(o->ceta);
+  o->ceta = NULL;
+  if (!eta) return;
+  const size_t cnt = (size_t)(o->n + 1) * o->k;
+  o->ceta = (float *)malloc(sizeof(float) * cnt);
+  memcpy(o->ceta, eta, sizeof(float) * cnt);
 }
 
 int oracle_cand_k(const OracleColony *o) { return o->k; }
@@ -448,7 +461,7 @@ int oracle_construct_ant(const OracleColony *o, int ant, int epoch, int32_t *tou
         dd[t] = state[j];
         if ((int)state[j] <= room) {
           uint8_t q = o->tau_cand[(size_t)cur * k + t];
-          w[t] = o->taupow[q] * o->eta_lut[d];
+          w[t] = o->taupow[q] * (o->ceta ? o->ceta[(size_t)cur * k + t] : o->eta_lut[d]);
         }
       }
       oracle_scan_rows(w, k, P);

@@ -33,6 +33,9 @@ void oracle_get_tables(const OracleColony *o, float *deq, float *thr, float *tau
 void oracle_get_tau(const OracleColony *o, uint8_t *tau, uint8_t *tau_cand);
 void oracle_set_tau(OracleColony *o, const uint8_t *tau);
 void oracle_set_epoch(OracleColony *o, int epoch);
+/* Per-candidate-slot heuristic ((n+1)*k values) replacing (1/d)^beta in the
+ * candidate roulette; NULL restores the distance heuristic. */
+void oracle_set_candidate_eta(OracleColony *o, const float *eta);
 void oracle_set_stop_rules(OracleColony *o, int stagnation_epochs, double min_rel,
                            int fixed_epochs);
 /* One ant's tour (tour capacity >= tour_stride). Returns the tour length. */

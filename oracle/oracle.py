@@ -41,6 +41,7 @@ _SIG = {
     "oracle_get_tau": (None, [C.c_void_p, u8p, u8p]),
     "oracle_set_tau": (None, [C.c_void_p, u8p]),
     "oracle_set_epoch": (None, [C.c_void_p, C.c_int]),
+    "oracle_set_candidate_eta": (None, [C.c_void_p, C.c_void_p]),
     "oracle_set_stop_rules": (None, [C.c_void_p, C.c_int, C.c_double, C.c_int]),
     "oracle_construct_ant": (C.c_int, [C.c_void_p, C.c_int, C.c_int, i32p, i64p, i32p, i64p]),
     "oracle_construct": (None, [C.c_void_p, C.c_int]),
@@ -148,6 +149,15 @@ class Oracle:
 
     def set_epoch(self, e):
         self.lib.oracle_set_epoch(self.h, e)
+
+    def set_candidate_eta(self, eta):
+        """Learned heuristic per candidate slot ((n+1, k) floats); None
+        restores (1/d)^beta."""
+        if eta is None:
+            self.lib.oracle_set_candidate_eta(self.h, None)
+            return
+        self._ceta = np.ascontiguousarray(eta, np.float32).reshape(-1)
+        self.lib.oracle_set_candidate_eta(self.h, self._ceta.ctypes.data_as(C.c_void_p))
 
     def set_stop_rules(self, stagnation=0, min_rel=1e-3, fixed=0):
         self.lib.oracle_set_stop_rules(self.h, stagnation, min_rel, fixed)

@@ -197,6 +197,34 @@ def test_fused_update_matches_oracle(gpu, name, monkeypatch):
     s.close()
 
 
+@pytest.mark.parametrize("name", ["c1_n100", "n300_clustered"])
+def test_learned_eta_matches_oracle(gpu, name):
+    """The learned-prior seam: a per-slot heuristic replaces (1/d)^beta in
+    the candidate roulette of the GPU path and of the oracle alike."""
+    inst, s, o = make_pair(name)
+    s.prepare()
+    k = s.cand_k(0)
+    rng = np.random.default_rng(17)
+    eta = rng.uniform(1e-6, 1e-3, size=(inst.n + 1, k)).astype(np.float32)
+    eta[rng.random(eta.shape) < 0.1] = 0.0  # some slots never drawn
+    s.set_candidate_eta(eta)
+    o.set_candidate_eta(eta)
+    for e in range(4):
+        s.epochs(1)
+        o.epochs(1)
+        _compare_tours(s, o)
+        assert np.array_equal(s.tau()[0], o.tau()[0]), f"tau differs after epoch {e}"
+        t = o.tables()
+        want = (t["taupow"][o.tau()[1]] * eta).astype(np.float32)
+        assert np.array_equal(s.scores().view(np.uint32), want.view(np.uint32))
+    s.set_candidate_eta(None)  # back to (1/d)^beta
+    t = o.tables()
+    _, dist, _ = o.candidates()
+    want = (t["taupow"][o.tau()[1]] * t["eta"][dist]).astype(np.float32)
+    assert np.array_equal(s.scores().view(np.uint32), want.view(np.uint32))
+    s.close()
+
+
 def test_ants_per_group_invariance(gpu):
     ref = None
     for apg in (1, 3, 8):

@@ -191,6 +191,28 @@ def test_construct_invariants_and_replay(seed):
     assert not np.array_equal(o3.tours()[0], tours)
 
 
+def test_learned_eta_seam():
+    """The learned-prior seam: a per-slot heuristic equal to the distance
+    term reproduces the distance mode bit for bit; an all-zero one leaves no
+    candidate weight, so every decision takes the all-customer fallback."""
+    inst = Instance.uniform(120, 6, seed=4)
+    base = O.Oracle(inst, cand_k=12, ants=16, rho=0.1, seed=4)
+    base.epochs(3)
+    o = O.Oracle(inst, cand_k=12, ants=16, rho=0.1, seed=4)
+    _, _, eta = o.candidates()
+    o.set_candidate_eta(eta)
+    o.epochs(3)
+    assert np.array_equal(o.tours()[0], base.tours()[0])
+    assert np.array_equal(o.tau()[0], base.tau()[0])
+    z = O.Oracle(inst, cand_k=12, ants=4, rho=0.1, seed=4)
+    z.set_candidate_eta(np.zeros_like(eta))
+    z.construct()
+    tours, lens, cost, unv = z.tours()
+    _check_tours(inst, tours, lens, cost, unv, z)
+    decisions = int(sum(lens) - (inst.K + 1) * len(lens))  # customer moves
+    assert z.fallback_count() >= decisions
+
+
 def test_unit_demand_always_feasible():
     # the reference close rule guarantees feasibility for unit demand
     inst = Instance.read(os.path.join(GOLDEN, "ref_n500_k8_s19000.vrp"), require_unit_demand=True)

@@ -66,6 +66,7 @@ struct AcoHipContext {
   uint8_t *d_slot = nullptr;
   float *d_score = nullptr;
   float *d_fscore = nullptr;
+  float *d_ceta = nullptr;
   uint8_t *d_pher = nullptr;
   int32_t *d_tours = nullptr, *d_len = nullptr, *d_unvis = nullptr, *d_best = nullptr;
   long long *d_cost = nullptr;
@@ -135,7 +136,7 @@ static void free_pools(AcoHipContext *c) {
                   c->d_dscore, c->d_rec, c->d_pher,   c->d_tours,    c->d_len,  c->d_unvis,
                   c->d_best, c->d_cost,  c->d_deq,    c->d_thr,      c->d_taupow, c->d_eta,
                   c->d_scratch_tour, c->d_hist, c->d_stats, c->d_cidx, c->d_score, c->d_done, c->d_slot,
-                  c->d_fscore};
+                  c->d_fscore, c->d_ceta};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_inst = nullptr; c->d_col = nullptr; c->d_state = nullptr; c->d_row_inst = nullptr;
@@ -144,7 +145,7 @@ static void free_pools(AcoHipContext *c) {
   c->d_best = nullptr; c->d_cost = nullptr; c->d_deq = nullptr; c->d_thr = nullptr;
   c->d_taupow = nullptr; c->d_eta = nullptr; c->d_scratch_tour = nullptr; c->d_hist = nullptr;
   c->d_stats = nullptr; c->d_cidx = nullptr; c->d_score = nullptr; c->d_done = nullptr;
-  c->d_slot = nullptr; c->d_fscore = nullptr;
+  c->d_slot = nullptr; c->d_fscore = nullptr; c->d_ceta = nullptr;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -483,6 +484,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if ((rc = dalloc(c, &c->d_dscore, sizeof(float) * c->nodes_total))) return rc;
   if ((rc = dalloc(c, &c->d_rec, sizeof(uint32_t) * c->rec_total))) return rc;
   if ((rc = dalloc(c, &c->d_cidx, sizeof(uint16_t) * c->rec_total))) return rc;
+  if ((rc = dalloc(c, &c->d_ceta, sizeof(float) * c->rec_total))) return rc;
   if ((rc = dalloc(c, &c->d_slot, c->slot_total))) return rc;
   if ((rc = dalloc(c, &c->d_score, sizeof(float) * c->score_total))) return rc;
   if (c->fs_on && (rc = dalloc(c, &c->d_fscore, sizeof(float) * c->fs_total))) return rc;
@@ -535,6 +537,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.slot_of = c->d_slot;
   P.score = c->d_score;
   P.fscore = c->fs_on ? c->d_fscore : nullptr;
+  P.ceta = c->d_ceta;
   P.pher = c->d_pher;
   P.tours = c->d_tours;
   P.len = c->d_len;
@@ -930,9 +933,30 @@ extern "C" int aco_hip_get_candidates(AcoHipContext *c, int inst, int32_t *idx, 
 
 extern "C" int aco_hip_set_candidate_eta(AcoHipContext *c, int inst, const float *eta) {
   NEED_LOADED(c);
-  (void)inst;
-  (void)eta;
-  return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "learned candidate heuristic not enabled");
+  if (inst < 0 || inst >= c->num_inst)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "instance %d out of range", inst);
+  InstDesc &d = c->inst[inst];
+  const size_t cnt = ((size_t)d.n + 1) * d.k;
+  if (eta) {
+    for (size_t e = 0; e < cnt; ++e)
+      if (!(eta[e] >= 0.0f && eta[e] <= 3.0e38f))
+        return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
+                       "candidate heuristic %zu is not a finite value >= 0", e);
+    HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, hipMemcpy(c->d_ceta + d.cand_off, eta, sizeof(float) * cnt, hipMemcpyHostToDevice));
+  } else {
+    HIPC(c, hipStreamSynchronize(c->stream));
+  }
+  d.learned = eta ? 1 : 0;
+  HIPC(c, hipMemcpy(c->d_inst + inst, &d, sizeof(InstDesc), hipMemcpyHostToDevice));
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  int rc = launch_refresh(c);
+  if (rc) return rc;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ACO_HIP_OK;
 }
 
 extern "C" int aco_hip_get_tau(AcoHipContext *c, int colony, uint8_t *tau, uint8_t *tau_cand) {

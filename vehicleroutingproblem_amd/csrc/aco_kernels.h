@@ -45,6 +45,8 @@ struct InstDesc {
   uint32_t seed;
   int fs_stride;       /* S = n+1 rounded up to 16: fscore row stride and the
                           per-ant state length in LDS */
+  int learned;         /* candidate slots weigh ceta[slot] instead of eta[d] */
+  int pad;
 };
 
 struct ColonyDesc {
@@ -76,6 +78,7 @@ struct DevCtx {
   uint8_t *slot_of;    /* per instance (n+1)^2: 1 + slot of v in row u, 0 if none */
   float *score;        /* taupow[tau_cand] * eta[d] per slot, per colony */
   float *fscore;       /* per colony (n+1) x S all-pairs weights, NULL = off */
+  const float *ceta;   /* learned heuristic per candidate slot (indexed like rec) */
   uint8_t *pher;
   int32_t *tours;
   int32_t *len;
@@ -116,6 +119,13 @@ struct HistEntry {
 };
 
 static inline __host__ __device__ size_t round16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+/* Heuristic factor of candidate slot e (0 <= e < (n+1)k) of instance I: the
+ * learned prior when set, else (1/d)^beta of the slot's nint distance. */
+static inline __device__ float slot_eta(const DevCtx &P, const InstDesc &I, size_t e,
+                                        uint32_t rec) {
+  return I.learned ? P.ceta[I.cand_off + e] : P.eta[rec >> 16];
+}
 
 /* is_significant_improvement (src/cuda/aco_cuda.cu:83-94), ACO_EPS = 1e-9. */
 static inline __host__ __device__ int aco_significant(double prev, double now, double min_rel) {

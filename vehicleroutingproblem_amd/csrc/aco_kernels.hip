@@ -688,7 +688,7 @@ __global__ void __launch_bounds__(256) k_evaporate(DevCtx P, int mat_blocks, int
     uint32_t q = tc[e];
     q = q > d ? q - d : 0u;
     tc[e] = (uint8_t)q;
-    score[e] = P.taupow[q] * P.eta[rec[e] >> 16];
+    score[e] = P.taupow[q] * slot_eta(P, I, e, rec[e]);
   }
 }
 
@@ -705,7 +705,7 @@ __global__ void __launch_bounds__(256) k_refresh_scores(DevCtx P) {
   const uint32_t *rec = P.rec + I.cand_off;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < slots;
        e += (size_t)gridDim.x * blockDim.x)
-    score[e] = P.taupow[tc[e]] * P.eta[rec[e] >> 16];
+    score[e] = P.taupow[tc[e]] * slot_eta(P, I, e, rec[e]);
   if (P.fscore)
     fs_items(P, D, I, P.taupow, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x,
              false, 0u);
@@ -828,14 +828,16 @@ __device__ __forceinline__ void patch_slot(const DevCtx &P, const UpdTables &T,
       const uint32_t r = row[t];
       if ((int)(r & 0xFFFFu) == v) {
         tc[(size_t)u * k + t] = val;
-        score[(size_t)u * k + t] = T.taupow[val] * T.eta[r >> 16];
+        score[(size_t)u * k + t] =
+            T.taupow[val] * (I.learned ? P.ceta[I.cand_off + (size_t)u * k + t] : T.eta[r >> 16]);
       }
     }
     return;
   }
   if (slot1) {
-    tc[(size_t)u * k + slot1 - 1] = val;
-    score[(size_t)u * k + slot1 - 1] = T.taupow[val] * T.eta[d];
+    const size_t e = (size_t)u * k + slot1 - 1;
+    tc[e] = val;
+    score[e] = T.taupow[val] * (I.learned ? P.ceta[I.cand_off + e] : T.eta[d]);
   }
 }
 
@@ -1260,7 +1262,7 @@ __global__ void __launch_bounds__(1024) k_evap_update(DevCtx P, int mat_blocks) 
         uint32_t q = tc[e];
         q = q > d ? q - d : 0u;
         tc[e] = (uint8_t)q;
-        score[e] = P.taupow[q] * P.eta[rec[e] >> 16];
+        score[e] = P.taupow[q] * slot_eta(P, I, e, rec[e]);
       }
     }
   }

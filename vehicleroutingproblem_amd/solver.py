@@ -291,6 +291,15 @@ class HipSolver:
                                                           N.ptr(out, C.c_float)))
         return out
 
+    def set_candidate_eta(self, eta, instance: int = 0):
+        """Learned heuristic per candidate slot ((n+1, k) floats >= 0) replacing
+        (1/d)^beta in the candidate roulette; None restores it."""
+        if eta is None:
+            self._check(self.lib.aco_hip_set_candidate_eta(self._ctx, instance, None))
+            return
+        e = np.ascontiguousarray(eta, np.float32)
+        self._check(self.lib.aco_hip_set_candidate_eta(self._ctx, instance, N.ptr(e, C.c_float)))
+
     def set_tau(self, tau: np.ndarray, colony: int = 0):
         tau = np.ascontiguousarray(tau, dtype=np.uint8)
         self._check(self.lib.aco_hip_set_tau(self._ctx, colony, N.ptr(tau, C.c_uint8)))
