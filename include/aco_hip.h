@@ -187,7 +187,8 @@ int aco_hip_epochs(AcoHipContext *ctx, int count);
 int aco_hip_run(AcoHipContext *ctx, const AcoHipRunConfig *cfg, AcoHipRunStats *stats);
 
 /* ---- downloads / uploads (synchronous) ---------------------------------- */
-/* (n+1)*k candidate indices, nint distances and eta^beta values (any may be NULL). */
+/* (n+1)*k candidate indices, nint distances and heuristic values (any may be
+ * NULL): eta^beta of the distance, or the learned heuristic once one is set. */
 int aco_hip_get_candidates(AcoHipContext *ctx, int instance, int32_t *idx, int32_t *dist,
                            float *eta);
 /* Learned heuristic prior (the seam of aco_cuda_kernels.cu:295, where eta_beta
@@ -248,6 +249,48 @@ typedef struct {
 int aco_hip_set_stats(AcoHipContext *ctx, int enable);
 /* Counters summed since enabling, for one colony. */
 int aco_hip_get_stats(AcoHipContext *ctx, int colony, AcoHipStats *st);
+
+/* ---- learned edge prior ----------------------------------------------------
+ * No reference counterpart: the edge-scoring network that replaces eta_beta
+ * at the seam of aco_cuda_kernels.cu:295 (see aco_hip_set_candidate_eta).
+ * Network 8 -> 64 -> 64 -> 1 (ReLU), BF16 matrices on MFMA with FP32
+ * accumulation, FP32 master parameters, Adam, mean binary cross-entropy
+ * against "candidate edge joins consecutive nodes of an elite tour".
+ * Parameter vector (ACO_PRIOR_PARAMS floats): W1[64][8], b1[64], W2[64][64],
+ * b2[64], w3[64], b3.  Features per candidate slot (i, t), j = cand[i][t],
+ * L = larger side of the coordinate bounding box: (x_i-x_0)/L, (y_i-y_0)/L,
+ * (x_j-x_0)/L, (y_j-y_0)/L, demand_j/Q, nint d(i,j)/L, cosine of the angle
+ * at i between i->j and i->depot, t/k; each rounded to BF16. */
+#define ACO_PRIOR_FEATURES 8
+#define ACO_PRIOR_PARAMS 4801
+typedef struct AcoHipPrior AcoHipPrior;
+/* Glorot-uniform weights from Philox(seed), zero biases. */
+int aco_hip_prior_create(AcoHipPrior **out, int device, uint32_t seed);
+void aco_hip_prior_destroy(AcoHipPrior *prior);
+const char *aco_hip_prior_last_error(const AcoHipPrior *prior);
+int aco_hip_prior_get_params(AcoHipPrior *prior, float *params);
+/* Also resets the optimiser state. */
+int aco_hip_prior_set_params(AcoHipPrior *prior, const float *params);
+/* (n+1)*k rows of ACO_PRIOR_FEATURES BF16 features (raw uint16) of an instance. */
+int aco_hip_edge_features(AcoHipContext *ctx, int instance, uint16_t *features);
+/* Logits of `count` edges given as BF16 feature rows. */
+int aco_hip_prior_forward(AcoHipPrior *prior, const uint16_t *features, int64_t count,
+                          float *logits);
+/* Gradient of the mean BCE (ACO_PRIOR_PARAMS floats) and the mean BCE itself. */
+int aco_hip_prior_gradient(AcoHipPrior *prior, const uint16_t *features, const uint8_t *labels,
+                           int64_t count, float *grad, float *loss);
+/* Training set: append every candidate edge of an instance, labelled from an
+ * elite tour of it (len entries, depot-delimited routes). */
+int aco_hip_prior_add_instance(AcoHipPrior *prior, AcoHipContext *ctx, int instance,
+                               const int32_t *tour, int len);
+int aco_hip_prior_clear(AcoHipPrior *prior);
+int64_t aco_hip_prior_size(const AcoHipPrior *prior);
+/* `steps` full-batch Adam steps (beta 0.9/0.999) over the training set;
+ * loss_out (may be NULL) = mean BCE of the last step's forward pass. */
+int aco_hip_prior_train(AcoHipPrior *prior, int steps, float lr, float *loss_out);
+/* Score an instance's candidate edges and install
+ * eta = (1/d)^beta * gain * sigmoid(logit) as its learned heuristic. */
+int aco_hip_prior_apply(AcoHipPrior *prior, AcoHipContext *ctx, int instance, float gain);
 
 /* ---- timing ------------------------------------------------------------- */
 int aco_hip_set_timing(AcoHipContext *ctx, int enable);

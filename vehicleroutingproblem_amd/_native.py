@@ -27,6 +27,7 @@ i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
 u8p = C.POINTER(C.c_uint8)
 f32p = C.POINTER(C.c_float)
+u16p = C.POINTER(C.c_uint16)
 
 
 class AcoHipInstance(C.Structure):
@@ -132,6 +133,19 @@ HIP_SYMBOLS = {
     "aco_vrp_hip_with_capacity": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, i32p, i32p, i32p,
                                             C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_double, C.c_uint, i32p, i32p, i64p]),
+    "aco_hip_prior_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_uint32]),
+    "aco_hip_prior_destroy": (None, [C.c_void_p]),
+    "aco_hip_prior_last_error": (C.c_char_p, [C.c_void_p]),
+    "aco_hip_prior_get_params": (C.c_int, [C.c_void_p, f32p]),
+    "aco_hip_prior_set_params": (C.c_int, [C.c_void_p, f32p]),
+    "aco_hip_edge_features": (C.c_int, [ctx_p, C.c_int, u16p]),
+    "aco_hip_prior_forward": (C.c_int, [C.c_void_p, u16p, C.c_int64, f32p]),
+    "aco_hip_prior_gradient": (C.c_int, [C.c_void_p, u16p, u8p, C.c_int64, f32p, f32p]),
+    "aco_hip_prior_add_instance": (C.c_int, [C.c_void_p, ctx_p, C.c_int, i32p, C.c_int]),
+    "aco_hip_prior_clear": (C.c_int, [C.c_void_p]),
+    "aco_hip_prior_size": (C.c_int64, [C.c_void_p]),
+    "aco_hip_prior_train": (C.c_int, [C.c_void_p, C.c_int, C.c_float, f32p]),
+    "aco_hip_prior_apply": (C.c_int, [C.c_void_p, ctx_p, C.c_int, C.c_float]),
 }
 
 HOST_SYMBOLS = {

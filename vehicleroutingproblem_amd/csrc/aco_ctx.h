@@ -1,0 +1,110 @@
+/*
+ * aco_ctx.h - the context object behind the C ABI (internal; shared by the
+ * solver host code aco_hip.hip and the learned-prior host code aco_prior.hip).
+ */
+#ifndef ACO_CTX_H
+#define ACO_CTX_H
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "aco_hip.h"
+#include "aco_host.h"
+#include "aco_kernels.h"
+
+#define LDS_LIMIT 163840u
+#define HIST ACO_HIST
+
+struct TimingRec {
+  int kind;
+  hipEvent_t a, b;
+};
+
+struct AcoHipContext {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char err[512] = {0};
+  bool loaded = false;
+  AcoHipParams prm{};
+  int num_inst = 0, num_col = 0, W = 4, stage = 1, groups = 1, total_rows = 0, eta_lds = 1;
+  int rows = 1;        /* 16-lane rows of the widest candidate list (k_construct ROWS) */
+  int fs_on = 0;       /* all-pairs fallback weight matrix allocated */
+  size_t sd_lds = 0;   /* dynamic LDS of k_select_deposit, 0 = fused k_evap_update path */
+  int sd_eta = 0;      /* k_select_deposit stages the eta table in LDS */
+  int sd_blocks = 1;   /* workgroups of k_select_deposit per colony */
+  size_t lds_bytes = 0;
+  std::vector<InstDesc> inst;
+  std::vector<ColonyDesc> col;
+  std::vector<int> inst_n1;
+  AcoTauTables tables{};
+  std::vector<float> eta_lut;
+  int eta_len = 0;
+  size_t pher_total = 0, tours_total = 0, best_total = 0, rec_total = 0, nodes_total = 0;
+  size_t score_total = 0, slot_total = 0, fs_total = 0;
+  int ants_total = 0, max_stride = 0, max_mat_vec = 0, max_slots = 0, max_fs_items = 0;
+  /* device pools */
+  InstDesc *d_inst = nullptr;
+  ColonyDesc *d_col = nullptr;
+  AcoHipColonyState *d_state = nullptr;
+  int *d_row_inst = nullptr;
+  int2 *d_xy = nullptr;
+  uint16_t *d_dem = nullptr;
+  float *d_dscore = nullptr;
+  uint32_t *d_rec = nullptr;
+  uint16_t *d_cidx = nullptr;
+  uint8_t *d_slot = nullptr;
+  float *d_score = nullptr;
+  float *d_fscore = nullptr;
+  float *d_ceta = nullptr;
+  uint8_t *d_pher = nullptr;
+  int32_t *d_tours = nullptr, *d_len = nullptr, *d_unvis = nullptr, *d_best = nullptr;
+  long long *d_cost = nullptr;
+  float *d_deq = nullptr, *d_thr = nullptr, *d_taupow = nullptr, *d_eta = nullptr;
+  int32_t *d_scratch_tour = nullptr;
+  HistEntry *d_hist = nullptr;
+  unsigned long long *d_stats = nullptr;
+  int *d_done = nullptr;
+  DevCtx P{};
+  /* timing */
+  bool timing = false;
+  std::vector<TimingRec> pending;
+  std::vector<hipEvent_t> free_events;
+  AcoHipTiming t{};
+  /* graph */
+  hipGraphExec_t graph = nullptr;
+  int graph_chunk = 0;
+};
+
+/* ---- helpers shared by the host files ---------------------------------------- */
+
+int set_err(AcoHipContext *c, int code, const char *fmt, ...);
+/* Recompute every colony's candidate scores (and all-pairs weights). */
+int launch_refresh(AcoHipContext *c);
+/* Switch instance `inst` to (learned != 0) or from the per-slot heuristic
+ * already in d_ceta, and refresh the scores (stream-ordered, synchronous). */
+int ctx_set_learned(AcoHipContext *c, int inst, int learned);
+
+#define HIPC(ctx, call)                                                                         \
+  do {                                                                                          \
+    hipError_t e__ = (call);                                                                    \
+    if (e__ != hipSuccess)                                                                      \
+      return set_err(ctx, ACO_HIP_ERR_BACKEND, "%s failed at %s:%d: %s", #call, __FILE__,      \
+                     __LINE__, hipGetErrorString(e__));                                         \
+  } while (0)
+
+#define LAUNCHED(ctx, name)                                                                     \
+  do {                                                                                          \
+    hipError_t e__ = hipGetLastError();                                                         \
+    if (e__ != hipSuccess)                                                                      \
+      return set_err(ctx, ACO_HIP_ERR_BACKEND, "launch of %s failed: %s", name,                 \
+                     hipGetErrorString(e__));                                                   \
+  } while (0)
+
+#define NEED_LOADED(c)                                                                          \
+  do {                                                                                          \
+    if (!(c)) return ACO_HIP_ERR_INVALID_INPUT;                                                 \
+    if (!(c)->loaded) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "no instance loaded");       \
+  } while (0)
+
+#endif

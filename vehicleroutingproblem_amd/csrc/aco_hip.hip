@@ -22,73 +22,11 @@
 #include "aco_host.h"
 #include "aco_inline.h"
 #include "aco_kernels.h"
-
-#define LDS_LIMIT 163840u
-#define HIST ACO_HIST
-
-struct TimingRec {
-  int kind;
-  hipEvent_t a, b;
-};
-
-struct AcoHipContext {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  char err[512] = {0};
-  bool loaded = false;
-  AcoHipParams prm{};
-  int num_inst = 0, num_col = 0, W = 4, stage = 1, groups = 1, total_rows = 0, eta_lds = 1;
-  int rows = 1;        /* 16-lane rows of the widest candidate list (k_construct ROWS) */
-  int fs_on = 0;       /* all-pairs fallback weight matrix allocated */
-  size_t sd_lds = 0;   /* dynamic LDS of k_select_deposit, 0 = fused k_evap_update path */
-  int sd_eta = 0;      /* k_select_deposit stages the eta table in LDS */
-  int sd_blocks = 1;   /* workgroups of k_select_deposit per colony */
-  size_t lds_bytes = 0;
-  std::vector<InstDesc> inst;
-  std::vector<ColonyDesc> col;
-  std::vector<int> inst_n1;
-  AcoTauTables tables{};
-  std::vector<float> eta_lut;
-  int eta_len = 0;
-  size_t pher_total = 0, tours_total = 0, best_total = 0, rec_total = 0, nodes_total = 0;
-  size_t score_total = 0, slot_total = 0, fs_total = 0;
-  int ants_total = 0, max_stride = 0, max_mat_vec = 0, max_slots = 0, max_fs_items = 0;
-  /* device pools */
-  InstDesc *d_inst = nullptr;
-  ColonyDesc *d_col = nullptr;
-  AcoHipColonyState *d_state = nullptr;
-  int *d_row_inst = nullptr;
-  int2 *d_xy = nullptr;
-  uint16_t *d_dem = nullptr;
-  float *d_dscore = nullptr;
-  uint32_t *d_rec = nullptr;
-  uint16_t *d_cidx = nullptr;
-  uint8_t *d_slot = nullptr;
-  float *d_score = nullptr;
-  float *d_fscore = nullptr;
-  float *d_ceta = nullptr;
-  uint8_t *d_pher = nullptr;
-  int32_t *d_tours = nullptr, *d_len = nullptr, *d_unvis = nullptr, *d_best = nullptr;
-  long long *d_cost = nullptr;
-  float *d_deq = nullptr, *d_thr = nullptr, *d_taupow = nullptr, *d_eta = nullptr;
-  int32_t *d_scratch_tour = nullptr;
-  HistEntry *d_hist = nullptr;
-  unsigned long long *d_stats = nullptr;
-  int *d_done = nullptr;
-  DevCtx P{};
-  /* timing */
-  bool timing = false;
-  std::vector<TimingRec> pending;
-  std::vector<hipEvent_t> free_events;
-  AcoHipTiming t{};
-  /* graph */
-  hipGraphExec_t graph = nullptr;
-  int graph_chunk = 0;
-};
+#include "aco_ctx.h"
 
 /* ---- helpers --------------------------------------------------------------- */
 
-static int set_err(AcoHipContext *c, int code, const char *fmt, ...) {
+int set_err(AcoHipContext *c, int code, const char *fmt, ...) {
   if (c) {
     va_list ap;
     va_start(ap, fmt);
@@ -97,22 +35,6 @@ static int set_err(AcoHipContext *c, int code, const char *fmt, ...) {
   }
   return code;
 }
-
-#define HIPC(ctx, call)                                                                         \
-  do {                                                                                          \
-    hipError_t e__ = (call);                                                                    \
-    if (e__ != hipSuccess)                                                                      \
-      return set_err(ctx, ACO_HIP_ERR_BACKEND, "%s failed at %s:%d: %s", #call, __FILE__,      \
-                     __LINE__, hipGetErrorString(e__));                                         \
-  } while (0)
-
-#define LAUNCHED(ctx, name)                                                                     \
-  do {                                                                                          \
-    hipError_t e__ = hipGetLastError();                                                         \
-    if (e__ != hipSuccess)                                                                      \
-      return set_err(ctx, ACO_HIP_ERR_BACKEND, "launch of %s failed: %s", name,                 \
-                     hipGetErrorString(e__));                                                   \
-  } while (0)
 
 static double now_s(void) {
   struct timespec ts;
@@ -172,7 +94,6 @@ static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on, i
   return b;
 }
 
-static int launch_refresh(AcoHipContext *c);
 
 /* ---- timing ---------------------------------------------------------------- */
 
@@ -321,6 +242,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     }
     dmax = std::max(dmax, (int)aco_nint_dist(xmin, ymin, xmax, ymax));
     InstDesc &d = c->inst[i];
+    d.scale = (float)std::max(1, std::max(xmax - xmin, ymax - ymin));
     d.n = a.n;
     d.K = a.K;
     d.k = std::min(p->cand_k, a.n);
@@ -586,11 +508,6 @@ extern "C" int aco_hip_get_shape(const AcoHipContext *c, int *cand_k, int *ants,
   return ACO_HIP_OK;
 }
 
-#define NEED_LOADED(c)                                                                          \
-  do {                                                                                          \
-    if (!(c)) return ACO_HIP_ERR_INVALID_INPUT;                                                 \
-    if (!(c)->loaded) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "no instance loaded");       \
-  } while (0)
 
 /* ---- kernels ----------------------------------------------------------------- */
 
@@ -709,7 +626,7 @@ static int launch_select_deposit(AcoHipContext *c) {
   return ACO_HIP_OK;
 }
 
-static int launch_refresh(AcoHipContext *c) {
+int launch_refresh(AcoHipContext *c) {
   const int work = std::max(c->max_slots, c->fs_on ? c->max_fs_items : 0);
   const int xs = std::max(1, std::min((work + 255) / 256, 1024));
   hipLaunchKernelGGL(k_refresh_scores, dim3(xs, c->num_col), dim3(256), 0, c->stream, c->P);
@@ -928,6 +845,9 @@ extern "C" int aco_hip_get_candidates(AcoHipContext *c, int inst, int32_t *idx, 
     if (dist) dist[e] = (int32_t)dd;
     if (eta) eta[e] = j ? c->eta_lut[dd] : 0.0f;
   }
+  /* a learned heuristic replaces the distance term: report the one in use */
+  if (eta && d.learned)
+    HIPC(c, hipMemcpy(eta, c->d_ceta + d.cand_off, cnt * sizeof(float), hipMemcpyDeviceToHost));
   return ACO_HIP_OK;
 }
 
@@ -947,7 +867,13 @@ extern "C" int aco_hip_set_candidate_eta(AcoHipContext *c, int inst, const float
   } else {
     HIPC(c, hipStreamSynchronize(c->stream));
   }
-  d.learned = eta ? 1 : 0;
+  return ctx_set_learned(c, inst, eta ? 1 : 0);
+}
+
+int ctx_set_learned(AcoHipContext *c, int inst, int learned) {
+  InstDesc &d = c->inst[inst];
+  d.learned = learned ? 1 : 0;
+  HIPC(c, hipStreamSynchronize(c->stream));
   HIPC(c, hipMemcpy(c->d_inst + inst, &d, sizeof(InstDesc), hipMemcpyHostToDevice));
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);

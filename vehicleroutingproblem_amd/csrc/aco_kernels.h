@@ -46,7 +46,7 @@ struct InstDesc {
   int fs_stride;       /* S = n+1 rounded up to 16: fscore row stride and the
                           per-ant state length in LDS */
   int learned;         /* candidate slots weigh ceta[slot] instead of eta[d] */
-  int pad;
+  float scale;         /* larger side of the coordinate bounding box (>= 1) */
 };
 
 struct ColonyDesc {
