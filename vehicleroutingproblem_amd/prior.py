@@ -107,3 +107,48 @@ def edge_features(solver: HipSolver, instance: int = 0) -> np.ndarray:
     solver._check(solver.lib.aco_hip_edge_features(solver._ctx, instance,
                                                    N.ptr(out, C.c_uint16)))
     return out
+
+
+def training_instances(count: int, nmin: int = 200, nmax: int = 2000, seed: int = 0):
+    """Seeded training instances: n ~ U[nmin, nmax], even indices from the
+    uniform generator, odd from the clustered one (25 Gaussian components,
+    sigma 30, in [0,1000]^2), K = ceil(n/125), demands U[1,10],
+    Q = ceil(1.1 * sum / K)."""
+    import math
+    from .solver import Instance
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        K = max(1, math.ceil(n / 125))
+        s = (seed * 100003 + i) & 0x7FFFFFFF
+        if i % 2 == 0:
+            out.append(Instance.uniform(n, K, side=1000, dmin=1, dmax=10, slack_pct=110, seed=s))
+        else:
+            out.append(Instance.clustered(n, K, side=1000, clusters=25, sigma=30, dmin=1, dmax=10,
+                                          slack_pct=110, seed=s))
+    return out
+
+
+def train_prior(instances, epochs: int = 100, steps: int = 600, lr: float = 2e-3,
+                cand_k: int = 16, ants: int = 64, batch: int = 64, seed: int = 1,
+                device: int = 0) -> "EdgePrior":
+    """Harvest elite tours with plain ACO (`batch` instances per context, one
+    colony each, `epochs` epochs) and train a prior on them on the GPU."""
+    from .solver import Params
+    prior = EdgePrior(seed=seed, device=device)
+    for b in range(0, len(instances), batch):
+        part = instances[b:b + batch]
+        with HipSolver(part, Params(cand_k=cand_k, ants=ants, rho=0.1),
+                       seeds=[(seed + b + i) & 0xFFFFFFFF for i in range(len(part))],
+                       device=device) as s:
+            s.prepare()
+            s.run(fixed_epochs=epochs, chunk_epochs=min(epochs, 32))
+            for i in range(len(part)):
+                try:
+                    tour, _ = s.best(i)
+                except AcoError:  # no feasible tour found: nothing to learn from
+                    continue
+                prior.add_instance(s, tour, instance=i)
+    prior.train(steps, lr=lr)
+    return prior
