@@ -97,6 +97,58 @@ def reference_openmp_baseline(seconds_budget=8.0):
                       f"candidate width), {t:.2f} s solver time"}
 
 
+def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
+    """Learned heuristic prior against plain ACO at equal wall-clock on
+    n=2000, K=16, 256 ants, k=16 instances (half uniform, half clustered):
+    the prior trains on elite tours of `train_count` seeded n in [200, 2000]
+    instances (harvested with plain ACO, 100 epochs, 64 ants), then each test
+    instance runs `budget_s` seconds with and without it (the learned run's
+    budget includes the prior's feature pass and inference)."""
+    import math
+    from vehicleroutingproblem_amd import HipSolver, Instance, Params
+    from vehicleroutingproblem_amd.prior import EdgePrior, train_prior, training_instances
+    t0 = time.perf_counter()
+    train = training_instances(train_count, seed=7)
+    prior = train_prior(train, epochs=100, steps=0)
+    t_harvest = time.perf_counter() - t0
+    edges = prior.size()
+    steps = 600
+    t0 = time.perf_counter()
+    prior.train(steps, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    # algorithmic flops per edge and step: forward 2(8*64 + 64*64 + 64), backward
+    # 2(64*64 [dW2] + 64*64 [dH1] + 64*8 [dW1] + 64 [dW3])
+    flops = edges * steps * (2 * (8 * 64 + 64 * 64 + 64) + 2 * (64 * 64 * 2 + 64 * 8 + 64))
+    plain, learned = [], []
+    for t in range(seeds):
+        if t % 2 == 0:
+            inst = Instance.uniform(2000, 16, side=1000, dmin=1, dmax=10, slack_pct=110,
+                                    seed=500 + t)
+        else:
+            inst = Instance.clustered(2000, 16, side=1000, clusters=25, sigma=30, dmin=1,
+                                      dmax=10, slack_pct=110, seed=500 + t)
+        for use in (False, True):
+            with HipSolver(inst, Params(cand_k=16, ants=256, rho=0.1), seeds=[77 + t]) as sv:
+                sv.prepare()
+                ts = time.perf_counter()
+                if use:
+                    prior.apply(sv)
+                sv.run(timeout_seconds=budget_s - (time.perf_counter() - ts), max_epochs=10 ** 6,
+                       chunk_epochs=16)
+                (learned if use else plain).append(sv.best()[1])
+    prior.close()
+    gains = [(a - b) / a for a, b in zip(plain, learned)]
+    return {"test": "n=2000 K=16 m=256 k=16, half uniform / half clustered (sigma 30)",
+            "budget_s": budget_s, "seeds": seeds,
+            "plain_best": plain, "learned_best": learned,
+            "mean_gain_pct": round(100 * sum(gains) / len(gains), 3),
+            "train_instances": train_count, "train_edges": edges,
+            "harvest_s": round(t_harvest, 3), "train_steps": steps,
+            "train_s": round(t_train, 3),
+            "train_tflops": round(flops / t_train / 1e12, 2),
+            "mfma_peak_tflops_bf16": 2500.0}
+
+
 def div_chunk(epochs: int, cap: int) -> int:
     """Largest divisor of `epochs` up to `cap`: graph replays of that many
     epochs end exactly on the last epoch."""
@@ -112,6 +164,10 @@ def main():
     ap.add_argument("--ants-per-group", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=64,
                     help="max epochs per graph replay (the largest divisor of --steps up to it)")
+    ap.add_argument("--no-prior", action="store_true",
+                    help="skip the learned-prior vs plain ACO leg")
+    ap.add_argument("--prior-seconds", type=float, default=3.0,
+                    help="wall-clock budget per test run of the learned-prior leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device twice (calibrates the HBM "
                          "byte counters of a profiler against a copy of known size)")
@@ -226,6 +282,8 @@ def main():
             for k in ("evap", "select", "deposit")} if dev["updates"] else None),
     }
     s.close()
+    if not args.no_prior:
+        out["learned_prior"] = learned_prior_leg(args.prior_seconds, 4, 64)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inst)
         ref = reference_openmp_baseline()
