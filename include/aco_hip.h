@@ -72,6 +72,7 @@ typedef struct {
   const int32_t *y;      /* n+1 coordinates, 0..32767 */
   const int32_t *demand; /* n+1 demands, demand[0] == 0, customers 0..32767 */
   uint32_t seed;         /* solver seed of this instance (Philox key) */
+  int cand_k;            /* candidate list length of this instance, 0 = AcoHipParams.cand_k */
 } AcoHipInstance;
 
 /* Solver parameters shared by every instance of a context.
@@ -154,6 +155,30 @@ int aco_hip_num_colonies(const AcoHipContext *ctx);
 /* Effective per-context sizes after clamping: k, ants, tour stride, ants/group, lds flag. */
 int aco_hip_get_shape(const AcoHipContext *ctx, int *cand_k, int *ants, int *tour_stride,
                       int *ants_per_group, int *stage_lds);
+
+/* ---- size buckets and autotuning (no reference counterpart: the reference
+ *      solves one instance per launch) ---------------------------------------
+ * Instances of a context are grouped by node count (n+1 up to 256, 512,
+ * 1024, ...; ACO_HIP_BUCKETS=0 keeps one group); each bucket has its own
+ * construct launch shape, and the buckets' launches run concurrently. */
+typedef struct {
+  int n1_min, n1_max;    /* node counts present in the bucket */
+  int instances, colonies;
+  int ants_per_group;    /* wavefronts (ants) per workgroup */
+  int stage_lds;         /* candidate rows staged in LDS (1) or read from global memory (0) */
+  int lds_bytes;         /* LDS per workgroup */
+  float tuned_ms;        /* construct time of the chosen shape (autotuner), 0 = not tuned */
+} AcoHipBucketInfo;
+int aco_hip_num_buckets(const AcoHipContext *ctx);
+int aco_hip_get_bucket(const AcoHipContext *ctx, int bucket, AcoHipBucketInfo *info);
+/* Force a bucket's shape (stage_lds -1 = automatic); the effective shape may
+ * have fewer ants per workgroup when LDS does not fit the request. */
+int aco_hip_set_bucket_shape(AcoHipContext *ctx, int bucket, int ants_per_group, int stage_lds);
+/* Time `launches` construct launches of every bucket for each ants-per-
+ * workgroup in {1, 2, 4, 8} and each staging mode that fits, apply the
+ * fastest per bucket and report them (info: max_buckets entries).  Colonies
+ * must not be halted; pheromone and colony state are left unchanged. */
+int aco_hip_autotune(AcoHipContext *ctx, int launches, AcoHipBucketInfo *info, int max_buckets);
 
 /* ---- kernels (one launch each, asynchronous on the context stream) ------ */
 /* kNN candidate lists: replaces launch_build_candidate_lists

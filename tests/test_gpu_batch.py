@@ -1,0 +1,87 @@
+"""GPU tests of batched solving: size buckets, per-instance candidate
+lengths, the launch-shape autotuner and its cache.  The tours of every
+instance of a batch equal those of the instance solved alone (the draws
+depend only on the instance seed, colony, epoch, ant and decision), so
+batching, bucketing and launch shapes are pure scheduling."""
+import os
+
+import numpy as np
+import pytest
+
+from vehicleroutingproblem_amd import HipSolver, Instance, Params
+from vehicleroutingproblem_amd import tuner
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch():
+    insts = [Instance.uniform(60, 2, seed=31), Instance.clustered(300, 4, side=1000, sigma=30,
+                                                                  dmin=1, dmax=10, seed=32),
+             Instance.uniform(700, 6, seed=33), Instance.clustered(1500, 12, side=1000, sigma=30,
+                                                                   dmin=1, dmax=10, seed=34)]
+    ks = [10, 16, 20, 12]
+    seeds = [301, 302, 303, 304]
+    return insts, ks, seeds
+
+
+def test_buckets_match_single_instance_runs(gpu):
+    insts, ks, seeds = _batch()
+    p = Params(cand_k=16, ants=32, colonies=2, rho=0.1)
+    b = HipSolver(insts, p, seeds=seeds, cand_k=ks)
+    bk = b.buckets()
+    assert len(bk) == 4
+    assert sum(x["instances"] for x in bk) == 4 and sum(x["colonies"] for x in bk) == 8
+    assert [b.cand_k(i) for i in range(4)] == ks
+    b.prepare()
+    b.epochs(3)
+    for i, inst in enumerate(insts):
+        single = HipSolver(inst, Params(cand_k=ks[i], ants=32, colonies=1, rho=0.1),
+                           seeds=[seeds[i]])
+        single.prepare()
+        single.epochs(3)
+        for x, y in zip(single.tours(0), b.tours(2 * i)):
+            assert np.array_equal(x, y)
+        assert np.array_equal(single.tau(0)[0], b.tau(2 * i)[0])
+        assert single.colony_state(0)["global_best_cost"] == b.colony_state(2 * i)["global_best_cost"]
+        single.close()
+    b.close()
+
+
+def test_autotune_is_pure_scheduling(gpu, tmp_path):
+    insts, ks, seeds = _batch()
+    p = Params(cand_k=16, ants=64, rho=0.1)
+    ref = HipSolver(insts, p, seeds=seeds, cand_k=ks)
+    ref.prepare()
+    ref.epochs(2)
+    s = HipSolver(insts, p, seeds=seeds, cand_k=ks)
+    s.prepare()
+    cache = str(tmp_path / "tune.json")
+    shapes = tuner.tune_shapes(s, cache=cache, launches=2)
+    assert all(x["tuned_ms"] > 0 for x in shapes)
+    assert os.path.exists(cache)
+    s.epochs(2)
+    for c in range(len(insts)):
+        for x, y in zip(ref.tours(c), s.tours(c)):
+            assert np.array_equal(x, y)
+    # a second solver takes the shapes from the cache without timing
+    s2 = HipSolver(insts, p, seeds=seeds, cand_k=ks)
+    s2.prepare()
+    shapes2 = tuner.tune_shapes(s2, cache=cache)
+    assert [(x["ants_per_group"], x["stage_lds"]) for x in shapes2] == \
+        [(x["ants_per_group"], x["stage_lds"]) for x in shapes]
+    assert all(x["tuned_ms"] == 0 for x in shapes2)
+    for h in (ref, s, s2):
+        h.close()
+
+
+def test_bucket_shape_requests(gpu):
+    insts, ks, seeds = _batch()
+    s = HipSolver(insts, Params(cand_k=16, ants=32, rho=0.1), seeds=seeds, cand_k=ks)
+    s.set_bucket_shape(0, 2, 1)
+    b0 = s.buckets()[0]
+    assert b0["ants_per_group"] == 2 and b0["stage_lds"] == 1
+    s.set_bucket_shape(3, 8, 0)
+    assert s.buckets()[3]["stage_lds"] == 0
+    with pytest.raises(Exception):
+        s.set_bucket_shape(0, 9, 1)
+    s.close()

@@ -32,7 +32,14 @@ u16p = C.POINTER(C.c_uint16)
 
 class AcoHipInstance(C.Structure):
     _fields_ = [("n", C.c_int), ("K", C.c_int), ("capacity", C.c_int),
-                ("x", i32p), ("y", i32p), ("demand", i32p), ("seed", C.c_uint32)]
+                ("x", i32p), ("y", i32p), ("demand", i32p), ("seed", C.c_uint32),
+                ("cand_k", C.c_int)]
+
+
+class AcoHipBucketInfo(C.Structure):
+    _fields_ = [("n1_min", C.c_int), ("n1_max", C.c_int), ("instances", C.c_int),
+                ("colonies", C.c_int), ("ants_per_group", C.c_int), ("stage_lds", C.c_int),
+                ("lds_bytes", C.c_int), ("tuned_ms", C.c_float)]
 
 
 class AcoHipParams(C.Structure):
@@ -133,6 +140,10 @@ HIP_SYMBOLS = {
     "aco_vrp_hip_with_capacity": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, i32p, i32p, i32p,
                                             C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_double, C.c_uint, i32p, i32p, i64p]),
+    "aco_hip_num_buckets": (C.c_int, [ctx_p]),
+    "aco_hip_get_bucket": (C.c_int, [ctx_p, C.c_int, C.POINTER(AcoHipBucketInfo)]),
+    "aco_hip_set_bucket_shape": (C.c_int, [ctx_p, C.c_int, C.c_int, C.c_int]),
+    "aco_hip_autotune": (C.c_int, [ctx_p, C.c_int, C.POINTER(AcoHipBucketInfo), C.c_int]),
     "aco_hip_prior_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_uint32]),
     "aco_hip_prior_destroy": (None, [C.c_void_p]),
     "aco_hip_prior_last_error": (C.c_char_p, [C.c_void_p]),

@@ -21,6 +21,17 @@ struct TimingRec {
   hipEvent_t a, b;
 };
 
+/* One size class of instances and its construct launch shape. */
+struct LaunchBucket {
+  int n1_lo = 0, n1_hi = 0;  /* node-count range of the class */
+  std::vector<int> insts;    /* instances of the class */
+  std::vector<int> cols;     /* their colonies */
+  int col_off = 0;           /* offset of `cols` in d_colmap */
+  int W = 4, stage = 1, eta_lds = 1, rows = 1, groups = 1;
+  size_t lds = 0;
+  float tuned_ms = 0.0f;     /* construct time measured by the autotuner, 0 = not tuned */
+};
+
 struct AcoHipContext {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -74,6 +85,11 @@ struct AcoHipContext {
   /* graph */
   hipGraphExec_t graph = nullptr;
   int graph_chunk = 0;
+  /* size buckets: one construct launch each, forked over streams */
+  std::vector<LaunchBucket> buckets;
+  int *d_colmap = nullptr;
+  std::vector<hipStream_t> bstreams; /* streams of buckets 1.. */
+  std::vector<hipEvent_t> bevents;   /* fork event, then one join event per extra bucket */
 };
 
 /* ---- helpers shared by the host files ---------------------------------------- */

@@ -68,6 +68,13 @@ static void free_pools(AcoHipContext *c) {
   c->d_taupow = nullptr; c->d_eta = nullptr; c->d_scratch_tour = nullptr; c->d_hist = nullptr;
   c->d_stats = nullptr; c->d_cidx = nullptr; c->d_score = nullptr; c->d_done = nullptr;
   c->d_slot = nullptr; c->d_fscore = nullptr; c->d_ceta = nullptr;
+  if (c->d_colmap) (void)hipFree(c->d_colmap);
+  c->d_colmap = nullptr;
+  for (hipStream_t st : c->bstreams) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : c->bevents) (void)hipEventDestroy(ev);
+  c->bstreams.clear();
+  c->bevents.clear();
+  c->buckets.clear();
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -196,6 +203,54 @@ static int env_int(const char *name, int dflt) {
   return (s && *s) ? atoi(s) : dflt;
 }
 
+/* Ants per workgroup and LDS staging of a bucket.  Preference: the
+ * requested ants per workgroup (W_req, clamped to 1..8 and the ant count),
+ * staged rows with the eta table, staged rows without it, then (unless
+ * staging was forced) global rows, then fewer ants per workgroup. */
+static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_stage) {
+  int W = std::max(1, std::min(8, std::min(W_req, c->prm.ants)));
+  int stage = want_stage == 0 ? 0 : 1;
+  int eta_in_lds = 1;
+  size_t need = 0;
+  int kmax = 1;
+  for (int i : lb.insts) kmax = std::max(kmax, c->inst[i].k);
+  auto recompute = [&]() {
+    need = 0;
+    for (int i : lb.insts)
+      need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0, W,
+                                    stage, c->fs_on, c->max_stride));
+  };
+  recompute();
+  if (need > LDS_LIMIT) {
+    eta_in_lds = 0;
+    recompute();
+  }
+  if (need > LDS_LIMIT && want_stage < 0) {
+    stage = 0;
+    eta_in_lds = 1;
+    recompute();
+    if (need > LDS_LIMIT) {
+      eta_in_lds = 0;
+      recompute();
+    }
+  }
+  while (need > LDS_LIMIT && W > 1) {
+    --W;
+    recompute();
+  }
+  if (need > LDS_LIMIT)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
+                   "instance too large for the LDS layout (%zu bytes needed, stage=%d)", need,
+                   stage);
+  lb.W = W;
+  lb.stage = stage;
+  lb.eta_lds = eta_in_lds;
+  lb.rows = (kmax + 15) / 16;
+  lb.groups = (c->prm.ants + W - 1) / W;
+  lb.lds = need;
+  return ACO_HIP_OK;
+}
+
 extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
                             const AcoHipParams *p) {
   if (!c || !in || num <= 0 || !p) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null argument");
@@ -245,7 +300,9 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     d.scale = (float)std::max(1, std::max(xmax - xmin, ymax - ymin));
     d.n = a.n;
     d.K = a.K;
-    d.k = std::min(p->cand_k, a.n);
+    if (a.cand_k < 0 || a.cand_k > 64)
+      return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "instance %d: cand_k must be in [0, 64]", i);
+    d.k = std::min(a.cand_k > 0 ? a.cand_k : p->cand_k, a.n);
     d.Q = a.capacity;
     d.node_off = (int)node_off;
     d.row_base = row_base;
@@ -328,53 +385,53 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if (want_fs < 0) want_fs = fs_bytes <= ((size_t)16 << 30) ? 1 : 0;
   c->fs_on = want_fs ? 1 : 0;
 
-  /* launch shape: ants per workgroup and LDS staging */
-  int W = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
-  W = std::max(1, std::min(8, std::min(W, p->ants)));
-  int want_stage = p->stage_lds;
-  if (want_stage < 0) want_stage = env_int("ACO_HIP_STAGE_LDS", -1);
-  int max_n1 = 0, max_k = 0;
-  size_t need_stage = 0, need_plain = 0;
-  for (int i = 0; i < num; ++i) {
-    max_n1 = std::max(max_n1, c->inst_n1[i]);
-    max_k = std::max(max_k, c->inst[i].k);
-  }
-  (void)need_stage;
-  (void)need_plain;
+  /* launch shapes: instances are grouped by node count into size buckets
+   * (n+1 up to 256, 512, 1024, ..., one bucket unless ACO_HIP_BUCKETS=0 merges
+   * them); each bucket gets its own ants per workgroup and LDS staging and
+   * its own construct launch */
+  int max_k = 0;
+  for (int i = 0; i < num; ++i) max_k = std::max(max_k, c->inst[i].k);
   c->rows = (max_k + 15) / 16;
-  int stage = want_stage == 0 ? 0 : 1;
-  int eta_in_lds = 1;
-  size_t need = 0;
-  auto recompute = [&]() {
-    need = 0;
-    for (int i = 0; i < num; ++i)
-      need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0,
-                                    W, stage, c->fs_on, c->max_stride));
-  };
-  /* preference: staged rows with the eta table, staged rows without it, then
-   * (unless staging was forced) global rows, then fewer ants per workgroup */
-  recompute();
-  if (need > LDS_LIMIT) {
-    eta_in_lds = 0;
-    recompute();
-  }
-  if (need > LDS_LIMIT && want_stage < 0) {
-    stage = 0;
-    eta_in_lds = 1;
-    recompute();
-    if (need > LDS_LIMIT) {
-      eta_in_lds = 0;
-      recompute();
+  {
+    const bool split = env_int("ACO_HIP_BUCKETS", 1) != 0;
+    std::vector<int> cls(num);
+    for (int i = 0; i < num; ++i) {
+      int lo = 256;
+      while (split && lo < c->inst_n1[i] && lo < (1 << 17)) lo <<= 1;
+      cls[i] = split ? lo : 0;
+    }
+    std::vector<int> order(num);
+    for (int i = 0; i < num; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a] < cls[b]; });
+    for (int q = 0; q < num; ++q) {
+      const int i = order[q];
+      if (c->buckets.empty() || cls[i] != c->buckets.back().n1_hi) {
+        LaunchBucket nb;
+        nb.n1_hi = cls[i];
+        nb.n1_lo = 1 << 30;
+        c->buckets.push_back(nb);
+      }
+      LaunchBucket &lb = c->buckets.back();
+      lb.insts.push_back(i);
+      lb.n1_lo = std::min(lb.n1_lo, c->inst_n1[i]);
+      for (int cc = 0; cc < p->colonies; ++cc) lb.cols.push_back(i * p->colonies + cc);
+    }
+    int off = 0;
+    for (LaunchBucket &lb : c->buckets) {
+      int hi = 0;
+      for (int i : lb.insts) hi = std::max(hi, c->inst_n1[i]);
+      lb.n1_hi = hi; /* the range actually present */
+      lb.col_off = off;
+      off += (int)lb.cols.size();
     }
   }
-  while (need > LDS_LIMIT && W > 1) {
-    --W;
-    recompute();
+  const int W0 = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
+  int want_stage = p->stage_lds;
+  if (want_stage < 0) want_stage = env_int("ACO_HIP_STAGE_LDS", -1);
+  for (LaunchBucket &lb : c->buckets) {
+    int rc = choose_shape(c, lb, W0, want_stage);
+    if (rc) return rc;
   }
-  if (need > LDS_LIMIT)
-    return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
-                   "instance too large for the LDS layout (%zu bytes needed, stage=%d)", need,
-                   stage);
   /* epochs as evaporate + multi-workgroup select/deposit while both tours
    * and their position tables fit in LDS */
   size_t sd = 0;
@@ -384,11 +441,11 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   c->sd_lds = (sd <= 131072 && c->max_stride <= 65536) ? sd : 0;
   if (env_int("ACO_HIP_FUSED_UPDATE", 0)) c->sd_lds = 0; /* force the fused update kernel */
   c->sd_blocks = std::max(1, std::min(64, (2 * c->max_stride + 255) / 256));
-  c->W = W;
-  c->stage = stage;
-  c->eta_lds = eta_in_lds;
-  c->groups = (p->ants + W - 1) / W;
-  c->lds_bytes = need;
+  c->W = c->buckets[0].W;
+  c->stage = c->buckets[0].stage;
+  c->eta_lds = c->buckets[0].eta_lds;
+  c->groups = c->buckets[0].groups;
+  c->lds_bytes = c->buckets[0].lds;
 
   /* device pools */
   int rc;
@@ -399,6 +456,22 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if ((rc = dalloc(c, &c->d_stats, sizeof(unsigned long long) * ACO_STATS * c->num_col)))
     return rc;
   if ((rc = dalloc(c, &c->d_done, sizeof(int) * c->num_col))) return rc;
+  if ((rc = dalloc(c, &c->d_colmap, sizeof(int) * c->num_col))) return rc;
+  {
+    std::vector<int> cm;
+    for (const LaunchBucket &lb : c->buckets) cm.insert(cm.end(), lb.cols.begin(), lb.cols.end());
+    HIPC(c, hipMemcpy(c->d_colmap, cm.data(), sizeof(int) * cm.size(), hipMemcpyHostToDevice));
+  }
+  for (size_t b = 1; b < c->buckets.size(); ++b) {
+    hipStream_t st = nullptr;
+    HIPC(c, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    c->bstreams.push_back(st);
+  }
+  for (size_t b = 0; b < c->buckets.size(); ++b) {
+    hipEvent_t ev = nullptr;
+    HIPC(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    c->bevents.push_back(ev);
+  }
   if ((rc = dalloc(c, &c->d_row_inst, sizeof(int) * (size_t)c->total_rows))) return rc;
   /* +16 entries: 16-node vector reads of the last instance stay in bounds */
   if ((rc = dalloc(c, &c->d_xy, sizeof(int2) * (c->nodes_total + 16)))) return rc;
@@ -497,6 +570,57 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
 
 extern "C" int aco_hip_num_colonies(const AcoHipContext *c) { return c ? c->num_col : 0; }
 
+extern "C" int aco_hip_num_buckets(const AcoHipContext *c) {
+  return (c && c->loaded) ? (int)c->buckets.size() : 0;
+}
+
+static void bucket_info(const LaunchBucket &lb, AcoHipBucketInfo *info) {
+  info->n1_min = lb.n1_lo;
+  info->n1_max = lb.n1_hi;
+  info->instances = (int)lb.insts.size();
+  info->colonies = (int)lb.cols.size();
+  info->ants_per_group = lb.W;
+  info->stage_lds = lb.stage;
+  info->lds_bytes = (int)lb.lds;
+  info->tuned_ms = lb.tuned_ms;
+}
+
+extern "C" int aco_hip_get_bucket(const AcoHipContext *c, int b, AcoHipBucketInfo *info) {
+  if (!c || !c->loaded || !info || b < 0 || b >= (int)c->buckets.size())
+    return ACO_HIP_ERR_INVALID_INPUT;
+  bucket_info(c->buckets[b], info);
+  return ACO_HIP_OK;
+}
+
+static void drop_graph(AcoHipContext *c) {
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+}
+
+static void sync_shape0(AcoHipContext *c) {
+  c->W = c->buckets[0].W;
+  c->stage = c->buckets[0].stage;
+  c->eta_lds = c->buckets[0].eta_lds;
+  c->groups = c->buckets[0].groups;
+  c->lds_bytes = c->buckets[0].lds;
+}
+
+extern "C" int aco_hip_set_bucket_shape(AcoHipContext *c, int b, int apg, int stage) {
+  if (!c || !c->loaded) return ACO_HIP_ERR_INVALID_INPUT;
+  if (b < 0 || b >= (int)c->buckets.size() || apg < 1 || apg > 8 || stage < -1 || stage > 1)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "bad bucket shape request");
+  LaunchBucket trial = c->buckets[b];
+  int rc = choose_shape(c, trial, apg, stage);
+  if (rc) return rc;
+  trial.tuned_ms = 0.0f;
+  c->buckets[b] = trial;
+  sync_shape0(c);
+  drop_graph(c);
+  return ACO_HIP_OK;
+}
+
 extern "C" int aco_hip_get_shape(const AcoHipContext *c, int *cand_k, int *ants, int *stride,
                                  int *apg, int *stage) {
   if (!c || !c->loaded) return ACO_HIP_ERR_INVALID_INPUT;
@@ -550,34 +674,111 @@ extern "C" int aco_hip_init_tau(AcoHipContext *c) {
 }
 
 template <int R>
-static void launch_construct_rows(AcoHipContext *c, dim3 grid, dim3 block, bool stats) {
-  if (c->stage && !stats)
-    hipLaunchKernelGGL((k_construct<true, false, R>), grid, block, c->lds_bytes, c->stream, c->P,
-                       c->groups, c->W);
+static void launch_construct_rows(AcoHipContext *c, const LaunchBucket &lb, hipStream_t st,
+                                  bool stats) {
+  DevCtx P = c->P;
+  P.eta_lds = lb.eta_lds;
+  const int *cols = c->d_colmap + lb.col_off;
+  dim3 grid((unsigned)(lb.cols.size() * lb.groups)), block((unsigned)(64 * lb.W));
+  if (lb.stage && !stats)
+    hipLaunchKernelGGL((k_construct<true, false, R>), grid, block, lb.lds, st, P, lb.groups,
+                       lb.W, cols);
   else if (!stats)
-    hipLaunchKernelGGL((k_construct<false, false, R>), grid, block, c->lds_bytes, c->stream, c->P,
-                       c->groups, c->W);
-  else if (c->stage)
-    hipLaunchKernelGGL((k_construct<true, true, R>), grid, block, c->lds_bytes, c->stream, c->P,
-                       c->groups, c->W);
+    hipLaunchKernelGGL((k_construct<false, false, R>), grid, block, lb.lds, st, P, lb.groups,
+                       lb.W, cols);
+  else if (lb.stage)
+    hipLaunchKernelGGL((k_construct<true, true, R>), grid, block, lb.lds, st, P, lb.groups,
+                       lb.W, cols);
   else
-    hipLaunchKernelGGL((k_construct<false, true, R>), grid, block, c->lds_bytes, c->stream, c->P,
-                       c->groups, c->W);
+    hipLaunchKernelGGL((k_construct<false, true, R>), grid, block, lb.lds, st, P, lb.groups,
+                       lb.W, cols);
 }
 
+static void launch_bucket(AcoHipContext *c, const LaunchBucket &lb, hipStream_t st, bool stats) {
+  switch (lb.rows) {
+    case 1: launch_construct_rows<1>(c, lb, st, stats); break;
+    case 2: launch_construct_rows<2>(c, lb, st, stats); break;
+    case 3: launch_construct_rows<3>(c, lb, st, stats); break;
+    default: launch_construct_rows<4>(c, lb, st, stats); break;
+  }
+}
+
+/* One construct launch per size bucket: bucket 0 on the context stream, the
+ * others on their own streams forked from it and joined back, so the
+ * buckets run side by side (also inside a captured graph). */
 static int launch_construct(AcoHipContext *c) {
   {
     Timed tm(c, T_CONSTRUCT);
-    dim3 grid((unsigned)(c->num_col * c->groups)), block((unsigned)(64 * c->W));
     const bool stats = c->P.stats != nullptr;
-    switch (c->rows) {
-      case 1: launch_construct_rows<1>(c, grid, block, stats); break;
-      case 2: launch_construct_rows<2>(c, grid, block, stats); break;
-      case 3: launch_construct_rows<3>(c, grid, block, stats); break;
-      default: launch_construct_rows<4>(c, grid, block, stats); break;
+    const size_t nb = c->buckets.size();
+    if (nb > 1) {
+      HIPC(c, hipEventRecord(c->bevents[0], c->stream));
+      for (size_t b = 1; b < nb; ++b) HIPC(c, hipStreamWaitEvent(c->bstreams[b - 1], c->bevents[0], 0));
     }
+    launch_bucket(c, c->buckets[0], c->stream, stats);
+    for (size_t b = 1; b < nb; ++b) launch_bucket(c, c->buckets[b], c->bstreams[b - 1], stats);
+    if (nb > 1)
+      for (size_t b = 1; b < nb; ++b) {
+        HIPC(c, hipEventRecord(c->bevents[b], c->bstreams[b - 1]));
+        HIPC(c, hipStreamWaitEvent(c->stream, c->bevents[b], 0));
+      }
   }
   LAUNCHED(c, "k_construct");
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_autotune(AcoHipContext *c, int launches, AcoHipBucketInfo *info,
+                                int max_buckets) {
+  NEED_LOADED(c);
+  if (launches < 1) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "launches must be >= 1");
+  HIPC(c, hipSetDevice(c->device));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  {
+    std::vector<AcoHipColonyState> st(c->num_col);
+    HIPC(c, hipMemcpy(st.data(), c->d_state, sizeof(AcoHipColonyState) * c->num_col,
+                      hipMemcpyDeviceToHost));
+    for (const AcoHipColonyState &x : st)
+      if (x.halted) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "autotune needs running colonies");
+  }
+  hipEvent_t a = nullptr, z = nullptr;
+  HIPC(c, hipEventCreate(&a));
+  HIPC(c, hipEventCreate(&z));
+  const bool stats = c->P.stats != nullptr;
+  for (size_t b = 0; b < c->buckets.size(); ++b) {
+    LaunchBucket best = c->buckets[b];
+    float best_ms = -1.0f;
+    for (int stage = 1; stage >= 0; --stage)
+      for (int W = 1; W <= 8; W <<= 1) {
+        if (W > c->prm.ants) continue;
+        LaunchBucket trial = c->buckets[b];
+        if (choose_shape(c, trial, W, stage) != ACO_HIP_OK) continue;
+        if (trial.W != W || trial.stage != stage) continue; /* does not fit as asked */
+        launch_bucket(c, trial, c->stream, stats);          /* warm caches */
+        HIPC(c, hipEventRecord(a, c->stream));
+        for (int r = 0; r < launches; ++r) launch_bucket(c, trial, c->stream, stats);
+        HIPC(c, hipEventRecord(z, c->stream));
+        LAUNCHED(c, "k_construct");
+        HIPC(c, hipEventSynchronize(z));
+        float ms = 0.0f;
+        HIPC(c, hipEventElapsedTime(&ms, a, z));
+        ms /= (float)launches;
+        if (best_ms < 0.0f || ms < best_ms) {
+          best_ms = ms;
+          best = trial;
+        }
+      }
+    if (best_ms >= 0.0f) {
+      best.tuned_ms = best_ms;
+      c->buckets[b] = best;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(z);
+  sync_shape0(c);
+  drop_graph(c);
+  if (info)
+    for (int b = 0; b < (int)c->buckets.size() && b < max_buckets; ++b)
+      bucket_info(c->buckets[b], info + b);
   return ACO_HIP_OK;
 }
 

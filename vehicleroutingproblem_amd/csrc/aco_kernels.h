@@ -140,7 +140,7 @@ static inline __host__ __device__ int aco_significant(double prev, double now, d
 __global__ void k_knn(DevCtx P, int total_rows, int knn_key, float depot_w);
 __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
 template <bool STAGE, bool STATS, int ROWS>
-__global__ void k_construct(DevCtx P, int groups_per_colony, int W);
+__global__ void k_construct(DevCtx P, int groups_per_colony, int W, const int *cols);
 __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
 __global__ void k_refresh_scores(DevCtx P);
 __global__ void k_update(DevCtx P, int mode);

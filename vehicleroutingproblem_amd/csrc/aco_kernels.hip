@@ -350,11 +350,14 @@ __device__ __forceinline__ float cand_scan(float v, int lane) {
  * select the move.  All control flow is uniform (scalar).  ROWS = 16-lane rows
  * the widest candidate list spans. */
 template <bool STAGE, bool STATS, int ROWS>
-__global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colony, int W) {
+__global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colony, int W,
+                                                   const int *cols) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const unsigned long long t_kernel = STATS ? clock64() : 0ull;
-  const int colony = blockIdx.x / groups_per_colony;
-  const int group = blockIdx.x - colony * groups_per_colony;
+  /* cols: the colonies of this launch (one launch per size bucket) */
+  const int slot = blockIdx.x / groups_per_colony;
+  const int group = blockIdx.x - slot * groups_per_colony;
+  const int colony = cols ? cols[slot] : slot;
   const ColonyDesc D = P.col[colony];
   const AcoHipColonyState *S = P.state + colony;
   if (S->halted) return; /* uniform over the workgroup */
@@ -562,10 +565,10 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
 }
 
 #define ACO_CONSTRUCT_VARIANTS(R)                                     \
-  template __global__ void k_construct<true, false, R>(DevCtx, int, int);  \
-  template __global__ void k_construct<false, false, R>(DevCtx, int, int); \
-  template __global__ void k_construct<true, true, R>(DevCtx, int, int);   \
-  template __global__ void k_construct<false, true, R>(DevCtx, int, int);
+  template __global__ void k_construct<true, false, R>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct<false, false, R>(DevCtx, int, int, const int *); \
+  template __global__ void k_construct<true, true, R>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct<false, true, R>(DevCtx, int, int, const int *);
 ACO_CONSTRUCT_VARIANTS(1)
 ACO_CONSTRUCT_VARIANTS(2)
 ACO_CONSTRUCT_VARIANTS(3)

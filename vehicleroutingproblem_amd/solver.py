@@ -147,7 +147,8 @@ class HipSolver:
     """One context of the C ABI: a batch of instances, each with `colonies`
     colonies of `ants` ants, resident on one GPU."""
 
-    def __init__(self, instances, params: Params | None = None, seeds=None, device: int = 0):
+    def __init__(self, instances, params: Params | None = None, seeds=None, device: int = 0,
+                 cand_k=None):
         self.lib = N.load()
         if isinstance(instances, Instance):
             instances = [instances]
@@ -156,6 +157,7 @@ class HipSolver:
         if seeds is None:
             seeds = [1234] * len(self.instances)
         self.seeds = [int(s) & 0xFFFFFFFF for s in seeds]
+        self._cand_k = [int(k) for k in cand_k] if cand_k is not None else None
         self._ctx = C.c_void_p()
         rc = self.lib.aco_hip_create(C.byref(self._ctx), device)
         if rc:
@@ -167,6 +169,7 @@ class HipSolver:
             arr[i].y = N.ptr(inst.y, C.c_int32)
             arr[i].demand = N.ptr(inst.demand, C.c_int32)
             arr[i].seed = seed
+            arr[i].cand_k = int(cand_k[i]) if cand_k is not None else 0
         self._check(self.lib.aco_hip_load(self._ctx, arr, len(self.instances),
                                           C.byref(self.params.to_c())))
         k, m, stride, apg, stage = (C.c_int32() for _ in range(5))
@@ -207,7 +210,8 @@ class HipSolver:
         return self.instances[colony // self.params.colonies]
 
     def cand_k(self, instance: int = 0) -> int:
-        return min(self.params.cand_k, self.instances[instance].n, 64)
+        k = self._cand_k[instance] if self._cand_k and self._cand_k[instance] else self.params.cand_k
+        return min(k, self.instances[instance].n, 64)
 
     def tour_stride(self, colony: int = 0) -> int:
         inst = self.instance_of(colony)
@@ -290,6 +294,29 @@ class HipSolver:
         self._check(self.lib.aco_hip_get_fallback_weights(self._ctx, colony,
                                                           N.ptr(out, C.c_float)))
         return out
+
+    # ---- size buckets and autotuning -------------------------------------------
+    @staticmethod
+    def _bucket_dict(b) -> dict:
+        return {f: getattr(b, f) for f, _ in b._fields_}
+
+    def buckets(self) -> list:
+        out = []
+        for i in range(self.lib.aco_hip_num_buckets(self._ctx)):
+            b = N.AcoHipBucketInfo()
+            self._check(self.lib.aco_hip_get_bucket(self._ctx, i, C.byref(b)))
+            out.append(self._bucket_dict(b))
+        return out
+
+    def set_bucket_shape(self, bucket: int, ants_per_group: int, stage_lds: int = -1):
+        self._check(self.lib.aco_hip_set_bucket_shape(self._ctx, bucket, ants_per_group,
+                                                      stage_lds))
+
+    def autotune(self, launches: int = 3) -> list:
+        nb = self.lib.aco_hip_num_buckets(self._ctx)
+        arr = (N.AcoHipBucketInfo * max(nb, 1))()
+        self._check(self.lib.aco_hip_autotune(self._ctx, launches, arr, nb))
+        return [self._bucket_dict(arr[i]) for i in range(nb)]
 
     def set_candidate_eta(self, eta, instance: int = 0):
         """Learned heuristic per candidate slot ((n+1, k) floats >= 0) replacing

@@ -1,0 +1,114 @@
+"""Autotuning for batched solves: launch shapes per size bucket and the
+candidate-list length per size class, cached on disk.
+
+* Shapes (ants per workgroup, LDS staging) are timed on the device by
+  aco_hip_autotune: every bucket's construct launch for each shape that fits.
+  Results are cached per (bucket node range, candidate length, ants).
+* The candidate length changes the search itself, so it is tuned on solution
+  quality: a sample instance of each size class is solved for a fixed
+  wall-clock budget with each candidate length, and the length with the best
+  cost wins (ties to the shorter list).  Cached per (size class, ants).
+"""
+import json
+import math
+import os
+import time
+
+from .solver import HipSolver, Params
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "aco_hip_tune.json")
+
+
+def size_class(n1: int) -> int:
+    """Upper node-count bound of the bucket an instance of n1 nodes joins
+    (the same classes as the library's size buckets)."""
+    c = 256
+    while c < n1:
+        c <<= 1
+    return c
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, table):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def _shape_key(bucket: dict, k: int, ants: int) -> str:
+    return f"shape|n1<={size_class(bucket['n1_max'])}|k{k}|m{ants}"
+
+
+def tune_shapes(solver: HipSolver, cache: str | None = DEFAULT_CACHE, launches: int = 3) -> list:
+    """Apply cached shapes to every bucket of `solver`; time the buckets with
+    no cache entry (the device tuner times all buckets) and store them."""
+    table = _load(cache) if cache else {}
+    ants = solver.ants
+
+    def bucket_k(info):
+        ks = [solver.cand_k(i) for i, inst in enumerate(solver.instances)
+              if info["n1_min"] <= inst.n + 1 <= info["n1_max"]]
+        return max(ks) if ks else solver.params.cand_k
+
+    buckets = solver.buckets()
+    missing = False
+    for b, info in enumerate(buckets):
+        hit = table.get(_shape_key(info, bucket_k(info), ants))
+        if hit:
+            solver.set_bucket_shape(b, hit["ants_per_group"], hit["stage_lds"])
+        else:
+            missing = True
+    if missing:
+        buckets = solver.autotune(launches)
+        for info in buckets:
+            table[_shape_key(info, bucket_k(info), ants)] = {"ants_per_group": info["ants_per_group"],
+                                                "stage_lds": info["stage_lds"],
+                                                "ms": info["tuned_ms"]}
+        if cache:
+            _store(cache, table)
+    return solver.buckets()
+
+
+def tune_candidate_length(samples: dict, ants: int, ks=(12, 16, 20, 24), seconds: float = 1.0,
+                          cache: str | None = DEFAULT_CACHE, seed: int = 1) -> dict:
+    """samples: {size class: Instance}.  Returns {size class: best k}."""
+    table = _load(cache) if cache else {}
+    out = {}
+    for cls, inst in sorted(samples.items()):
+        key = f"k|n1<={cls}|m{ants}"
+        if key in table:
+            out[cls] = table[key]["k"]
+            continue
+        best = None
+        for k in ks:
+            if k > inst.n:
+                continue
+            with HipSolver(inst, Params(cand_k=k, ants=ants, rho=0.1), seeds=[seed]) as s:
+                s.prepare()
+                t0 = time.perf_counter()
+                s.run(timeout_seconds=seconds, max_epochs=10 ** 6, chunk_epochs=16)
+                try:
+                    cost = s.best()[1]
+                except Exception:
+                    cost = math.inf
+            if best is None or cost < best[1]:
+                best = (k, cost)
+        out[cls] = best[0]
+        table[key] = {"k": best[0], "cost": best[1], "seconds": seconds}
+    if cache:
+        _store(cache, table)
+    return out
+
+
+def candidate_lengths(instances, k_by_class: dict, default: int = 16) -> list:
+    """Per-instance candidate lengths from a {size class: k} table."""
+    return [int(k_by_class.get(size_class(inst.n + 1), default)) for inst in instances]
