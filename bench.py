@@ -149,6 +149,53 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
             "mfma_peak_tflops_bf16": 2500.0}
 
 
+def batched_leg(epochs: int, count: int = 512, ants: int = 128):
+    """C4: `count` instances in one context (n ~ U[50, 2000], half uniform,
+    half clustered with sigma 30, K = ceil(n/125), demands U[1,10],
+    Q = ceil(1.1 sum/K)), `ants` ants each, per-instance seeds, candidate
+    lengths from the tuned per-size-class table, launch shapes autotuned on
+    the device; `epochs` timed epochs in graph replays after 2 warmup."""
+    import json as _json
+    from vehicleroutingproblem_amd import HipSolver, Params
+    from vehicleroutingproblem_amd.prior import training_instances
+    from vehicleroutingproblem_amd import tuner
+    insts = training_instances(count, nmin=50, nmax=2000, seed=4242)
+    table_path = os.path.join(ROOT, "vehicleroutingproblem_amd", "tuned_k.json")
+    k_table = {}
+    if os.path.exists(table_path):
+        k_table = {int(c): k for c, k in _json.load(open(table_path))["k_by_size_class"].items()}
+    ks = tuner.candidate_lengths(insts, k_table, default=16)
+    t0 = time.perf_counter()
+    s = HipSolver(insts, Params(cand_k=16, ants=ants, rho=0.1),
+                  seeds=[9000 + i for i in range(count)], cand_k=ks)
+    s.prepare()
+    s.sync()
+    t_prep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    shapes = tuner.tune_shapes(s, cache=None, launches=2)
+    t_tune = time.perf_counter() - t0
+    s.run(fixed_epochs=2, chunk_epochs=2, use_graph=True)
+    s.sync()
+    chunk = div_chunk(epochs, 64)
+    t0 = time.perf_counter()
+    s.run(fixed_epochs=2 + epochs, chunk_epochs=chunk, use_graph=True)
+    s.sync()
+    dt = time.perf_counter() - t0
+    done = s.colony_state(0)["epoch"] - 2
+    feasible = sum(1 for c in range(count) if s.colony_state(c)["global_best_cost"] < 2 ** 62)
+    s.close()
+    return {"workload": f"{count} instances n~U[50,2000] (half uniform, half clustered sigma 30), "
+                        f"K=ceil(n/125), {ants} ants each",
+            "value": round(count * ants * done / dt, 1), "unit": "ant-tours/s",
+            "epochs": done, "ms_per_epoch": round(1e3 * dt / max(done, 1), 3),
+            "instances_with_feasible_tour": feasible,
+            "cand_k_by_size_class": {str(c): k for c, k in sorted(k_table.items())} or "16",
+            "buckets": [{"n1": [b["n1_min"], b["n1_max"]], "instances": b["instances"],
+                         "ants_per_group": b["ants_per_group"], "stage_lds": b["stage_lds"],
+                         "construct_ms": round(b["tuned_ms"], 3)} for b in shapes],
+            "prepare_s": round(t_prep, 3), "autotune_s": round(t_tune, 3)}
+
+
 def div_chunk(epochs: int, cap: int) -> int:
     """Largest divisor of `epochs` up to `cap`: graph replays of that many
     epochs end exactly on the last epoch."""
@@ -164,6 +211,8 @@ def main():
     ap.add_argument("--ants-per-group", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=64,
                     help="max epochs per graph replay (the largest divisor of --steps up to it)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched 512-instance leg")
     ap.add_argument("--no-prior", action="store_true",
                     help="skip the learned-prior vs plain ACO leg")
     ap.add_argument("--prior-seconds", type=float, default=3.0,
@@ -282,6 +331,8 @@ def main():
             for k in ("evap", "select", "deposit")} if dev["updates"] else None),
     }
     s.close()
+    if not args.no_batch:
+        out["batched"] = batched_leg(20)
     if not args.no_prior:
         out["learned_prior"] = learned_prior_leg(args.prior_seconds, 4, 64)
     if not args.no_cpu:

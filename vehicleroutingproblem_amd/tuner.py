@@ -112,3 +112,33 @@ def tune_candidate_length(samples: dict, ants: int, ks=(12, 16, 20, 24), seconds
 def candidate_lengths(instances, k_by_class: dict, default: int = 16) -> list:
     """Per-instance candidate lengths from a {size class: k} table."""
     return [int(k_by_class.get(size_class(inst.n + 1), default)) for inst in instances]
+
+
+def main(argv=None):
+    """python -m vehicleroutingproblem_amd.tuner --k-table OUT.json: tune the
+    candidate length per size class on sample instances of the batched
+    generator (n ~ middle of each class) and write the table."""
+    import argparse
+    from .prior import training_instances
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--k-table", required=True)
+    ap.add_argument("--ants", type=int, default=128)
+    ap.add_argument("--seconds", type=float, default=1.5)
+    ap.add_argument("--ks", default="10,12,16,20,24")
+    a = ap.parse_args(argv)
+    ks = tuple(int(x) for x in a.ks.split(","))
+    samples = {}
+    for cls, n in ((256, 200), (512, 400), (1024, 900), (2048, 1800)):
+        samples[cls] = training_instances(1, nmin=n, nmax=n, seed=cls)[0]
+    table = tune_candidate_length(samples, a.ants, ks=ks, seconds=a.seconds, cache=None)
+    out = {"k_by_size_class": {str(c): k for c, k in sorted(table.items())},
+           "method": f"best cost after {a.seconds} s of plain ACO ({a.ants} ants) per candidate "
+                     f"length in {list(ks)}, one uniform sample instance per size class",
+           "command": "python -m vehicleroutingproblem_amd.tuner --k-table " + a.k_table}
+    with open(a.k_table, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
