@@ -80,30 +80,36 @@ def tune_shapes(solver: HipSolver, cache: str | None = DEFAULT_CACHE, launches: 
 
 def tune_candidate_length(samples: dict, ants: int, ks=(12, 16, 20, 24), seconds: float = 1.0,
                           cache: str | None = DEFAULT_CACHE, seed: int = 1) -> dict:
-    """samples: {size class: Instance}.  Returns {size class: best k}."""
+    """samples: {size class: [Instance, ...]}.  Each instance is solved for
+    `seconds` with each candidate length; a length's score is its cost
+    relative to the best length on that instance, averaged over the class's
+    instances; the lowest score wins (ties to the shorter list).  Returns
+    {size class: best k}."""
     table = _load(cache) if cache else {}
     out = {}
-    for cls, inst in sorted(samples.items()):
+    for cls, insts in sorted(samples.items()):
         key = f"k|n1<={cls}|m{ants}"
         if key in table:
             out[cls] = table[key]["k"]
             continue
-        best = None
-        for k in ks:
-            if k > inst.n:
-                continue
-            with HipSolver(inst, Params(cand_k=k, ants=ants, rho=0.1), seeds=[seed]) as s:
-                s.prepare()
-                t0 = time.perf_counter()
-                s.run(timeout_seconds=seconds, max_epochs=10 ** 6, chunk_epochs=16)
-                try:
-                    cost = s.best()[1]
-                except Exception:
-                    cost = math.inf
-            if best is None or cost < best[1]:
-                best = (k, cost)
-        out[cls] = best[0]
-        table[key] = {"k": best[0], "cost": best[1], "seconds": seconds}
+        usable = [k for k in ks if all(k <= inst.n for inst in insts)]
+        costs = {k: [] for k in usable}
+        for inst in insts:
+            for k in usable:
+                with HipSolver(inst, Params(cand_k=k, ants=ants, rho=0.1), seeds=[seed]) as s:
+                    s.prepare()
+                    s.run(timeout_seconds=seconds, max_epochs=10 ** 6, chunk_epochs=16)
+                    try:
+                        costs[k].append(float(s.best()[1]))
+                    except Exception:
+                        costs[k].append(math.inf)
+        score = {}
+        for k in usable:
+            rel = [c / min(costs[q][i] for q in usable) for i, c in enumerate(costs[k])]
+            score[k] = sum(rel) / len(rel)
+        best = min(usable, key=lambda k: (score[k], k))
+        out[cls] = best
+        table[key] = {"k": best, "score": {str(k): score[k] for k in usable}, "seconds": seconds}
     if cache:
         _store(cache, table)
     return out
@@ -129,11 +135,12 @@ def main(argv=None):
     ks = tuple(int(x) for x in a.ks.split(","))
     samples = {}
     for cls, n in ((256, 200), (512, 400), (1024, 900), (2048, 1800)):
-        samples[cls] = training_instances(1, nmin=n, nmax=n, seed=cls)[0]
+        samples[cls] = training_instances(2, nmin=n, nmax=n, seed=cls)  # uniform + clustered
     table = tune_candidate_length(samples, a.ants, ks=ks, seconds=a.seconds, cache=None)
     out = {"k_by_size_class": {str(c): k for c, k in sorted(table.items())},
            "method": f"best cost after {a.seconds} s of plain ACO ({a.ants} ants) per candidate "
-                     f"length in {list(ks)}, one uniform sample instance per size class",
+                     f"length in {list(ks)}, mean relative cost over a uniform and a clustered "
+                     "sample instance per size class",
            "command": "python -m vehicleroutingproblem_amd.tuner --k-table " + a.k_table}
     with open(a.k_table, "w") as f:
         json.dump(out, f, indent=1)
