@@ -88,6 +88,9 @@ struct AcoHipContext {
   /* size buckets: one construct launch each, forked over streams */
   std::vector<LaunchBucket> buckets;
   int *d_colmap = nullptr;
+  int *d_feas = nullptr;                   /* per colony: 1 + last epoch with a feasible ant */
+  long long *d_item_off = nullptr, *d_slot_off = nullptr; /* flat evaporation prefix offsets */
+  long long flat_total = 0;
   std::vector<hipStream_t> bstreams; /* streams of buckets 1.. */
   std::vector<hipEvent_t> bevents;   /* fork event, then one join event per extra bucket */
 };

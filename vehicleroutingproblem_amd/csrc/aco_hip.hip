@@ -70,6 +70,11 @@ static void free_pools(AcoHipContext *c) {
   c->d_slot = nullptr; c->d_fscore = nullptr; c->d_ceta = nullptr;
   if (c->d_colmap) (void)hipFree(c->d_colmap);
   c->d_colmap = nullptr;
+  if (c->d_feas) (void)hipFree(c->d_feas);
+  if (c->d_item_off) (void)hipFree(c->d_item_off);
+  if (c->d_slot_off) (void)hipFree(c->d_slot_off);
+  c->d_feas = nullptr;
+  c->d_item_off = c->d_slot_off = nullptr;
   for (hipStream_t st : c->bstreams) (void)hipStreamDestroy(st);
   for (hipEvent_t ev : c->bevents) (void)hipEventDestroy(ev);
   c->bstreams.clear();
@@ -457,6 +462,24 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     return rc;
   if ((rc = dalloc(c, &c->d_done, sizeof(int) * c->num_col))) return rc;
   if ((rc = dalloc(c, &c->d_colmap, sizeof(int) * c->num_col))) return rc;
+  if ((rc = dalloc(c, &c->d_feas, sizeof(int) * c->num_col))) return rc;
+  if ((rc = dalloc(c, &c->d_item_off, sizeof(long long) * (c->num_col + 1)))) return rc;
+  if ((rc = dalloc(c, &c->d_slot_off, sizeof(long long) * (c->num_col + 1)))) return rc;
+  {
+    std::vector<long long> io(c->num_col + 1, 0), so(c->num_col + 1, 0);
+    for (int q = 0; q < c->num_col; ++q) {
+      const ColonyDesc &cd = c->col[q];
+      const InstDesc &d = c->inst[cd.inst];
+      const long long n1 = d.n + 1;
+      io[q + 1] = io[q] + (c->fs_on ? n1 * (d.fs_stride / 16) : cd.mat_bytes / 16);
+      so[q + 1] = so[q] + n1 * d.k;
+    }
+    c->flat_total = io[c->num_col] + so[c->num_col];
+    HIPC(c, hipMemcpy(c->d_item_off, io.data(), sizeof(long long) * io.size(),
+                      hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->d_slot_off, so.data(), sizeof(long long) * so.size(),
+                      hipMemcpyHostToDevice));
+  }
   {
     std::vector<int> cm;
     for (const LaunchBucket &lb : c->buckets) cm.insert(cm.end(), lb.cols.begin(), lb.cols.end());
@@ -532,6 +555,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.slot_of = c->d_slot;
   P.score = c->d_score;
   P.fscore = c->fs_on ? c->d_fscore : nullptr;
+  P.feas = c->d_feas;
   P.ceta = c->d_ceta;
   P.pher = c->d_pher;
   P.tours = c->d_tours;
@@ -669,6 +693,7 @@ extern "C" int aco_hip_init_tau(AcoHipContext *c) {
   }
   HIPC(c, hipMemcpyAsync(c->d_state, st.data(), sizeof(AcoHipColonyState) * st.size(),
                          hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemsetAsync(c->d_feas, 0, sizeof(int) * c->num_col, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   return ACO_HIP_OK;
 }
@@ -817,6 +842,20 @@ static int launch_evap_update(AcoHipContext *c) {
   return ACO_HIP_OK;
 }
 
+static int launch_evap_flat(AcoHipContext *c) {
+  const long long blocks = std::max(1LL, std::min((c->flat_total + 255) / 256, 16384LL));
+  {
+    Timed tm(c, T_EVAP);
+    /* the taupow table, and the eta table while it is at most 32 KB, in LDS */
+    const int eta_lds = c->eta_len <= 8192 ? 1 : 0;
+    const size_t lds = 256 * 4 + (eta_lds ? (size_t)c->eta_len * 4 : 0);
+    hipLaunchKernelGGL(k_evaporate_flat, dim3((unsigned)blocks), dim3(256), lds, c->stream, c->P,
+                       c->d_item_off, c->d_slot_off, c->num_col, eta_lds);
+  }
+  LAUNCHED(c, "k_evaporate_flat");
+  return ACO_HIP_OK;
+}
+
 static int launch_select_deposit(AcoHipContext *c) {
   {
     Timed tm(c, T_BEST);
@@ -865,7 +904,7 @@ static int launch_epochs(AcoHipContext *c, int count) {
      * evaporate + single-workgroup update kernel instead */
     if ((rc = launch_construct(c))) return rc;
     if (c->sd_lds) {
-      if ((rc = launch_evap(c, 1))) return rc;
+      if ((rc = launch_evap_flat(c))) return rc;
       if ((rc = launch_select_deposit(c))) return rc;
     } else {
       if ((rc = launch_evap_update(c))) return rc;
@@ -1119,6 +1158,7 @@ extern "C" int aco_hip_set_tau(AcoHipContext *c, int colony, const uint8_t *tau)
   HIPC(c, hipMemcpy2D(c->d_pher + cd.pher_off, (size_t)d.fs_stride, tau, n1, n1, n1,
                       hipMemcpyHostToDevice));
   HIPC(c, hipMemcpy(c->d_pher + cd.tcand_off, tc.data(), cnt, hipMemcpyHostToDevice));
+  HIPC(c, hipMemset(c->d_feas + colony, 0, sizeof(int)));
   int rc2 = launch_refresh(c);
   if (rc2) return rc2;
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -1190,6 +1230,7 @@ extern "C" int aco_hip_set_epoch(AcoHipContext *c, int colony, int epoch) {
   if (rc) return rc;
   st.epoch = epoch;
   HIPC(c, hipMemcpy(c->d_state + colony, &st, sizeof(st), hipMemcpyHostToDevice));
+  HIPC(c, hipMemset(c->d_feas + colony, 0, sizeof(int))); /* no feasible ant seen at this epoch */
   return ACO_HIP_OK;
 }
 

@@ -100,6 +100,7 @@ struct DevCtx {
   void *hist;          /* HistEntry[colonies][ACO_HIST] per-epoch progress ring */
   unsigned long long *stats; /* [colonies][ACO_STATS] counters, NULL = off */
   int *done;           /* per-colony block counter of k_evap_update (kept at 0) */
+  int *feas;           /* per colony: 1 + the last epoch with a feasible ant */
 };
 
 /* counters per colony: construct [0] decisions, [1] fallbacks, [2] fallback
@@ -142,6 +143,8 @@ __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
 template <bool STAGE, bool STATS, int ROWS>
 __global__ void k_construct(DevCtx P, int groups_per_colony, int W, const int *cols);
 __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
+__global__ void k_evaporate_flat(DevCtx P, const long long *item_off, const long long *slot_off,
+                                 int ncol, int eta_lds);
 __global__ void k_refresh_scores(DevCtx P);
 __global__ void k_update(DevCtx P, int mode);
 __global__ void k_evap_update(DevCtx P, int mat_blocks);

@@ -553,6 +553,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
     P.len[a] = pos + 1;
     P.cost[a] = cost;
     P.unvis[a] = rem_cnt;
+    if (rem_cnt == 0) P.feas[colony] = epoch + 1; /* some ant of this epoch is feasible */
     if (STATS) {
       unsigned long long *sc = P.stats + (size_t)colony * ACO_STATS;
       atomicAdd(sc + 0, (unsigned long long)s);
@@ -596,47 +597,61 @@ __device__ __forceinline__ uint32_t evap_word(uint32_t x, uint32_t d) {
  * q - delta : 0 per byte) and four 16-byte weight stores.  `taupow` may be an
  * LDS copy of the table.  Coordinates of a group are read as 16-byte pairs;
  * the coordinate pool is padded by 16 entries past the last instance. */
-__device__ __forceinline__ void fs_items(const DevCtx &P, const ColonyDesc &D, const InstDesc &I,
-                                         const float *taupow, uint32_t first, uint32_t step,
-                                         bool evap, uint32_t dq) {
+__device__ __forceinline__ void fs_item(const DevCtx &P, const ColonyDesc &D, const InstDesc &I,
+                                        const float *taupow, const float *eta, uint32_t it,
+                                        bool evap, uint32_t dq) {
   const int n1 = I.n + 1, S = I.fs_stride;
   const uint32_t G = (uint32_t)S >> 4;
-  const uint32_t items = (uint32_t)n1 * G;
   uint8_t *__restrict__ tau = P.pher + D.pher_off;
   float *__restrict__ fs = P.fscore + D.fs_off;
   const int2 *__restrict__ xy = P.xy + I.node_off;
-  for (uint32_t it = first; it < items; it += step) {
-    const uint32_t u = it / G;
-    const int v0 = (int)(it - u * G) * 16;
-    const size_t row = (size_t)u * S + v0;
-    uint4 t = *reinterpret_cast<const uint4 *>(tau + row);
-    if (evap) {
-      t.x = evap_word(t.x, dq);
-      t.y = evap_word(t.y, dq);
-      t.z = evap_word(t.z, dq);
-      t.w = evap_word(t.w, dq);
-      *reinterpret_cast<uint4 *>(tau + row) = t;
-    }
-    const int2 pu = xy[u];
-    const int4 *x4 = reinterpret_cast<const int4 *>(xy + v0);
-    int4 q[8];
-#pragma unroll
-    for (int h = 0; h < 8; ++h) q[h] = x4[h];
-    const uint32_t tw[4] = {t.x, t.y, t.z, t.w};
-    float o[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int px = (i & 1) ? q[i >> 1].z : q[i >> 1].x;
-      const int py = (i & 1) ? q[i >> 1].w : q[i >> 1].y;
-      const uint32_t b = (tw[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-      const int d = aco_nint_dist(pu.x, pu.y, px, py);
-      o[i] = v0 + i < n1 ? taupow[b] * P.eta[v0 + i < n1 ? d : 0] : 0.0f;
-    }
-    float4 *f4 = reinterpret_cast<float4 *>(fs + row);
-#pragma unroll
-    for (int h = 0; h < 4; ++h)
-      f4[h] = make_float4(o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]);
+  const uint32_t u = it / G;
+  const int v0 = (int)(it - u * G) * 16;
+  const size_t row = (size_t)u * S + v0;
+  uint4 t = *reinterpret_cast<const uint4 *>(tau + row);
+  if (evap) {
+    /* levels at 0 stay at 0: an all-zero group keeps its weights */
+    if ((t.x | t.y | t.z | t.w) == 0u) return;
+    t.x = evap_word(t.x, dq);
+    t.y = evap_word(t.y, dq);
+    t.z = evap_word(t.z, dq);
+    t.w = evap_word(t.w, dq);
+    *reinterpret_cast<uint4 *>(tau + row) = t;
   }
+  const int2 pu = xy[u];
+  const int4 *x4 = reinterpret_cast<const int4 *>(xy + v0);
+  int4 q[8];
+#pragma unroll
+  for (int h = 0; h < 8; ++h) q[h] = x4[h];
+  const uint32_t tw[4] = {t.x, t.y, t.z, t.w};
+  float o[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int px = (i & 1) ? q[i >> 1].z : q[i >> 1].x;
+    const int py = (i & 1) ? q[i >> 1].w : q[i >> 1].y;
+    const uint32_t b = (tw[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+    const int d = aco_nint_dist(pu.x, pu.y, px, py);
+    o[i] = v0 + i < n1 ? taupow[b] * eta[v0 + i < n1 ? d : 0] : 0.0f;
+  }
+  float4 *f4 = reinterpret_cast<float4 *>(fs + row);
+#pragma unroll
+  for (int h = 0; h < 4; ++h)
+    f4[h] = make_float4(o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]);
+}
+
+/* Items of the all-pairs weight matrix: item it covers row u = it / G and
+ * the 16 columns 16g..16g+15 (G = S/16 items per row).  fscore[u][v] =
+ * taupow[tau[u][v]] * eta[nint d(u,v)], 0 past the last node.  Pheromone and
+ * weight rows share the 16-aligned stride S, so an item is one 16-byte
+ * pheromone load (and, with evap, one evaporated 16-byte store: q > delta ?
+ * q - delta : 0 per byte) and four 16-byte weight stores.  `taupow` may be an
+ * LDS copy of the table.  Coordinates of a group are read as 16-byte pairs;
+ * the coordinate pool is padded by 16 entries past the last instance. */
+__device__ __forceinline__ void fs_items(const DevCtx &P, const ColonyDesc &D, const InstDesc &I,
+                                         const float *taupow, uint32_t first, uint32_t step,
+                                         bool evap, uint32_t dq) {
+  const uint32_t items = (uint32_t)(I.n + 1) * ((uint32_t)I.fs_stride >> 4);
+  for (uint32_t it = first; it < items; it += step) fs_item(P, D, I, taupow, P.eta, it, evap, dq);
 }
 
 /* Saturating subtract (kernel_evaporate_tau, aco_cuda_kernels.cu:145-155).
@@ -692,6 +707,69 @@ __global__ void __launch_bounds__(256) k_evaporate(DevCtx P, int mat_blocks, int
     q = q > d ? q - d : 0u;
     tc[e] = (uint8_t)q;
     score[e] = P.taupow[q] * slot_eta(P, I, e, rec[e]);
+  }
+}
+
+/* Largest c in [0, n) with off[c] <= g (off ascending, off[0] = 0). */
+__device__ __forceinline__ int colony_of(const long long *off, int n, long long g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+/* Evaporation of every colony of the context in one flat grid: items
+ * [0, item_off[ncol]) are the colonies' matrix items (all-pairs weight items
+ * of fs_item, or 16-byte pheromone vectors without the weight matrix),
+ * items after them the candidate slots; each thread finds its colony by a
+ * binary search of the prefix offsets.  A colony evaporates iff it is not
+ * halted and some ant of this epoch was feasible (P.feas, tagged with the
+ * epoch by k_construct) - the condition of k_evaporate with from_ants. */
+__global__ void __launch_bounds__(256) k_evaporate_flat(DevCtx P, const long long *item_off,
+                                                        const long long *slot_off, int ncol,
+                                                        int eta_lds) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char dyn_smem[];
+  float *s_tp = reinterpret_cast<float *>(dyn_smem);
+  float *s_eta = s_tp + 256;
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) s_tp[i] = P.taupow[i];
+  if (eta_lds)
+    for (int i = threadIdx.x; i < P.eta_len; i += blockDim.x) s_eta[i] = P.eta[i];
+  __syncthreads();
+  const float *eta = eta_lds ? (const float *)s_eta : P.eta;
+  const long long ti = item_off[ncol], ts = slot_off[ncol];
+  const uint32_t d = P.q_evap;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < ti + ts;
+       g += (long long)gridDim.x * blockDim.x) {
+    const bool mat = g < ti;
+    const long long gg = mat ? g : g - ti;
+    const int c = colony_of(mat ? item_off : slot_off, ncol, gg);
+    const long long e = gg - (mat ? item_off[c] : slot_off[c]);
+    const AcoHipColonyState *S = P.state + c;
+    if (S->halted || P.feas[c] != S->epoch + 1) continue;
+    const ColonyDesc D = P.col[c];
+    const InstDesc I = P.inst[D.inst];
+    if (mat) {
+      if (P.fscore) {
+        fs_item(P, D, I, s_tp, eta, (uint32_t)e, true, d);
+      } else {
+        uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
+        uint4 v = base[e];
+        v.x = evap_word(v.x, d);
+        v.y = evap_word(v.y, d);
+        v.z = evap_word(v.z, d);
+        v.w = evap_word(v.w, d);
+        base[e] = v;
+      }
+    } else {
+      uint8_t *tc = P.pher + D.tcand_off;
+      uint32_t q = tc[e];
+      q = q > d ? q - d : 0u;
+      tc[e] = (uint8_t)q;
+      P.score[D.score_off + e] = s_tp[q] * slot_eta(P, I, e, P.rec[I.cand_off + e]);
+    }
   }
 }
 
