@@ -455,6 +455,16 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
     int cur = 0;
     int load = 0;
     const int close_cap = (K - veh - 1) * Q; /* close allowed iff rem_dem <= close_cap */
+    /* the current node's row: lane t holds candidate t (index, score, state)
+     * and every lane the node's depot score.  After a move the next row is
+     * loaded at once, ahead of the close roulette and the bookkeeping: none
+     * of its states can change before its decision, since the only state
+     * written meanwhile is the move's own, which is not in its candidate
+     * list (lanes past k read other rows and are masked out). */
+    int j = (int)c_idx[lane];
+    float sc = c_score[lane];
+    float dsc_cur = c_dsc[0];
+    uint16_t st = s_state[j]; /* 16-bit loop variable: widened at its use */
     /* aco_cuda_kernels.cu:492: the route goes on while rem_cnt > 0 and
      * load < Q; cur = -1 marks a route closed by a roulette (one exit test) */
     while (((rem_cnt - 1) | (Q - 1 - load) | cur) >= 0) {
@@ -471,12 +481,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
       const float u0 = f_readlane(rf0, sl);
 
       /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322) */
-      const int e = cur * k + lane;
-      const int j = (int)c_idx[e];
-      const float sc = c_score[e];
-      const float dsc_cur = c_dsc[cur]; /* issued with the first batch of loads */
-      const int st = (int)s_state[j];
-      const float w = (st <= room) ? sc : 0.0f;
+      const float w = ((int)st <= room) ? sc : 0.0f;
       const uint64_t emask = ballot64(w > 0.0f) & kmask;
       const float Pf = cand_scan<ROWS>(w, lane);
       int move, mdem;
@@ -488,7 +493,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
         const uint64_t m = ballot64(Pf > thr) & emask;
         const int sel = m ? first_lane(m) : last_lane(emask);
         move = __builtin_amdgcn_readlane(j, sel);
-        mdem = __builtin_amdgcn_readlane(st, sel);
+        mdem = __builtin_amdgcn_readlane((int)st, sel);
       } else {
         const unsigned long long f0 = STATS ? clock64() : 0ull;
         const FbOut fb = fallback_select(fsm ? fsm + (size_t)cur * Sn : nullptr, c_xy, c_taupow,
@@ -502,6 +507,12 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
           ++fallbacks;
         }
       }
+      /* prefetch of the next row (row 0 when nothing was found) */
+      const int mv = move > 0 ? move : 0;
+      const int e2 = mv * k + lane;
+      const int j2 = (int)c_idx[e2];
+      const float sc2 = c_score[e2];
+      const float dsc2 = c_dsc[mv];
       /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand;
        * depot scores are finite or 0 (never NaN), so dsc > 0 is a test of
        * the sign-and-magnitude bits.  A fallback that found nothing (move
@@ -514,6 +525,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
         }
       }
       if (move > 0) {
+        const uint16_t st2 = s_state[j2]; /* read before state[move] changes: j2 != move */
         ++pos;
         if (lane == 0) {
           s_state[move] = (uint16_t)VISITED;
@@ -526,6 +538,10 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
         rem_dem -= mdem;
         --rem_cnt;
         cur = move;
+        j = j2;
+        sc = sc2;
+        dsc_cur = dsc2;
+        st = st2;
       } else {
         cur = -1; /* nothing fits (fallback found none) or the depot closes */
       }
