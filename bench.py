@@ -97,6 +97,11 @@ def reference_openmp_baseline(seconds_budget=8.0):
                       f"candidate width), {t:.2f} s solver time"}
 
 
+# algorithmic flops per edge and training step: forward 2(8*64 + 64*64 + 64),
+# backward 2(64*64 [dW2] + 64*64 [dH1] + 64*8 [dW1] + 64 [dW3])
+PRIOR_FLOPS_PER_EDGE_STEP = 2 * (8 * 64 + 64 * 64 + 64) + 2 * (64 * 64 * 2 + 64 * 8 + 64)
+
+
 def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
     """Learned heuristic prior against plain ACO at equal wall-clock on
     n=2000, K=16, 256 ants, k=16 instances (half uniform, half clustered):
@@ -116,9 +121,7 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
     t0 = time.perf_counter()
     prior.train(steps, lr=2e-3)
     t_train = time.perf_counter() - t0
-    # algorithmic flops per edge and step: forward 2(8*64 + 64*64 + 64), backward
-    # 2(64*64 [dW2] + 64*64 [dH1] + 64*8 [dW1] + 64 [dW3])
-    flops = edges * steps * (2 * (8 * 64 + 64 * 64 + 64) + 2 * (64 * 64 * 2 + 64 * 8 + 64))
+    flops = edges * steps * PRIOR_FLOPS_PER_EDGE_STEP
     plain, learned = [], []
     for t in range(seeds):
         if t % 2 == 0:
@@ -196,6 +199,116 @@ def batched_leg(epochs: int, count: int = 512, ants: int = 128):
             "prepare_s": round(t_prep, 3), "autotune_s": round(t_tune, 3)}
 
 
+def large_clustered_leg(epochs: int = 20):
+    """C3: n=5000 clustered (25 Gaussian components, centres U[0,10000]^2,
+    sigma 300, depot at the centre), demands U[1,100], K=40,
+    Q=ceil(1.05 sum/40), 1024 ants, k=32; `epochs` timed epochs after 3."""
+    from vehicleroutingproblem_amd import HipSolver, Instance, Params
+    from vehicleroutingproblem_amd import tuner
+    inst = Instance.clustered(5000, 40, side=10000, clusters=25, sigma=300, dmin=1, dmax=100,
+                              slack_pct=105, seed=1)
+    with HipSolver(inst, Params(cand_k=32, ants=1024, rho=0.1), seeds=[1]) as s:
+        s.prepare()
+        shapes = tuner.tune_shapes(s, cache=None, launches=2)
+        s.run(fixed_epochs=3, chunk_epochs=3)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(fixed_epochs=3 + epochs, chunk_epochs=div_chunk(epochs, 64))
+        s.sync()
+        dt = time.perf_counter() - t0
+        done = s.colony_state()["epoch"] - 3
+        best = s.colony_state()["global_best_cost"]
+    return {"workload": "n=5000 K=40 clustered (sigma 300 in [0,10000]^2), demands U[1,100], "
+                        "Q=ceil(1.05 sum/40), 1024 ants, k=32",
+            "value": round(1024 * done / dt, 1), "unit": "ant-tours/s",
+            "ms_per_epoch": round(1e3 * dt / max(done, 1), 3), "epochs": done,
+            "best_cost_after": best,
+            "ants_per_group": shapes[0]["ants_per_group"], "stage_lds": shapes[0]["stage_lds"]}
+
+
+def real_tour_cost(inst, tour) -> float:
+    """Sum of real (double) Euclidean edge lengths along a depot-delimited
+    tour: the cost the reference CLI prints."""
+    import numpy as np
+    x = np.asarray(inst.x, np.float64)[tour]
+    y = np.asarray(inst.y, np.float64)[tour]
+    return float(np.hypot(np.diff(x), np.diff(y)).sum())
+
+
+def tour_is_feasible(inst, tour) -> bool:
+    """Starts and ends at the depot, every customer exactly once, every
+    route's load <= Q, at most K routes."""
+    import numpy as np
+    t = np.asarray(tour, np.int64)
+    if len(t) < 2 or t[0] != 0 or t[-1] != 0:
+        return False
+    cust = np.sort(t[t != 0])
+    if len(cust) != inst.n or (cust != np.arange(1, inst.n + 1)).any():
+        return False
+    routes, load = 0, 0
+    for a, b in zip(t[:-1], t[1:]):
+        if b == 0:
+            if a != 0:
+                routes += 1
+            load = 0
+        else:
+            load += int(inst.demand[b])
+            if load > inst.capacity:
+                return False
+    return routes <= inst.K
+
+
+def quality_leg(seeds: int = 32, n: int = 100, ants: int = 64):
+    """Solution quality against the reference's sequential solver at equal
+    epochs: unit-demand uniform instances (the reference parser's format),
+    n customers, K=8, Q=ceil(1.2 n/8); the reference sequential CLI (built
+    from its sources) runs to its stagnation stop (50 epochs) and reports
+    its epochs and best cost; this solver then runs exactly as many epochs
+    (alpha 1, beta 2, rho 0.5, tau0 1, Q 1, k=32, both CLIs' defaults) and its
+    best tour is costed in real (double) distances like the reference's."""
+    import re
+    import subprocess
+    import tempfile
+    import numpy as np
+    from vehicleroutingproblem_amd import HipSolver, Instance, Params
+    exe = os.path.join(ROOT, "oracle", "_ref", "aco_seq")
+    if not os.path.exists(exe):
+        return None
+    ref, ours, epochs = [], [], []
+    feasible = True
+    with tempfile.TemporaryDirectory() as td:
+        for i in range(seeds):
+            inst = Instance.uniform(n, 8, side=1000, dmin=1, dmax=1, slack_pct=120, seed=700 + i)
+            path = os.path.join(td, f"u{i}.vrp")
+            inst.write(path)
+            env = dict(os.environ, ACO_SOLVER_STAGNATION_EPOCHS="50",
+                       ACO_SOLVER_TIMEOUT_SECONDS="120", ACO_SOLVER_PROGRESS_INTERVAL_SECONDS="0")
+            out = subprocess.run([exe, path, "8", str(ants), str(1234 + i)], env=env,
+                                 capture_output=True, text=True, timeout=300).stdout
+            m = re.search(r"best cost: ([0-9.]+)", out)
+            ep = [int(x) for x in re.findall(r"epoca=([0-9]+)", out)]
+            if not m or not ep:
+                return None
+            ref.append(float(m.group(1)))
+            epochs.append(max(ep))
+            with HipSolver(inst, Params(cand_k=32, ants=ants, alpha=1.0, beta=2.0, rho=0.5,
+                                        tau0=1.0, Q=1.0), seeds=[1234 + i]) as s:
+                s.prepare()
+                s.run(fixed_epochs=max(ep), chunk_epochs=div_chunk(max(ep), 64))
+                tour, _ = s.best()
+            feasible &= tour_is_feasible(inst, tour)
+            ours.append(real_tour_cost(inst, tour))
+    diff = [(a - b) / b for a, b in zip(ours, ref)]
+    return {"workload": f"{seeds} unit-demand uniform instances, n={n}, K=8, Q=ceil(1.2n/8), "
+                        f"{ants} ants; reference sequential solver to its stagnation stop (50), "
+                        "this solver for the same number of epochs",
+            "mean_best_cost_diff_pct": round(100 * sum(diff) / len(diff), 3),
+            "all_tours_feasible": feasible,
+            "mean_epochs": round(sum(epochs) / len(epochs), 1),
+            "reference_mean_cost": round(sum(ref) / len(ref), 3),
+            "mean_cost": round(sum(ours) / len(ours), 3)}
+
+
 def div_chunk(epochs: int, cap: int) -> int:
     """Largest divisor of `epochs` up to `cap`: graph replays of that many
     epochs end exactly on the last epoch."""
@@ -212,7 +325,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=64,
                     help="max epochs per graph replay (the largest divisor of --steps up to it)")
     ap.add_argument("--no-batch", action="store_true",
-                    help="skip the batched 512-instance leg")
+                    help="skip the batched 512-instance and n=5000 legs")
+    ap.add_argument("--no-quality", action="store_true",
+                    help="skip the quality comparison with the reference sequential solver")
     ap.add_argument("--no-prior", action="store_true",
                     help="skip the learned-prior vs plain ACO leg")
     ap.add_argument("--prior-seconds", type=float, default=3.0,
@@ -220,7 +335,22 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device twice (calibrates the HBM "
                          "byte counters of a profiler against a copy of known size)")
+    ap.add_argument("--train-probe", type=int, default=0, metavar="STEPS",
+                    help="only harvest 16 training instances and train the edge prior "
+                         "STEPS steps (the network kernels alone, for counter runs)")
     args = ap.parse_args()
+    if args.train_probe:
+        from vehicleroutingproblem_amd.prior import train_prior, training_instances
+        prior = train_prior(training_instances(16, seed=7), epochs=30, steps=0)
+        prior.train(2, lr=2e-3)  # warm-up; train() returns after the device is done
+        t0 = time.perf_counter()
+        prior.train(args.train_probe, lr=2e-3)
+        dt = time.perf_counter() - t0
+        flops = prior.size() * args.train_probe * PRIOR_FLOPS_PER_EDGE_STEP
+        print(json.dumps({"train_probe_steps": args.train_probe, "edges": prior.size(),
+                          "train_s": round(dt, 4), "train_tflops": round(flops / dt / 1e12, 2)}))
+        prior.close()
+        return
     if args.copy_probe:
         import torch
         a = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
@@ -333,6 +463,9 @@ def main():
     s.close()
     if not args.no_batch:
         out["batched"] = batched_leg(20)
+        out["large_clustered"] = large_clustered_leg(20)
+    if not args.no_quality:
+        out["quality_vs_reference_seq"] = quality_leg(32)
     if not args.no_prior:
         out["learned_prior"] = learned_prior_leg(args.prior_seconds, 4, 64)
     if not args.no_cpu:

@@ -17,7 +17,7 @@
  * straight from D-layout registers; H1 (layer 2) and dZ2 (dH1 = dZ2 W2) are
  * transposed through 2 KB of LDS.  Gradients are summed per wavefront in
  * registers over its tiles, written as one row per wavefront, and reduced in
- * a fixed order (deterministic) by the update kernel.
+ * a fixed order in two stages (deterministic) before the update.
  */
 #include <hip/hip_runtime.h>
 
@@ -49,6 +49,7 @@
 #define NBW 4608
 
 #define MLP_GRID 1024 /* wavefronts of the gradient kernel (rows of partial sums) */
+#define RED_SPLIT 32  /* first-stage splits of the row reduction */
 
 typedef short bf4 __attribute__((ext_vector_type(4)));
 typedef float fl4 __attribute__((ext_vector_type(4)));
@@ -65,13 +66,17 @@ __host__ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+/* four FP32 -> BF16 (round to nearest even, as f2bf) with the hardware's
+ * packed conversion, two values per instruction */
 __device__ __forceinline__ bf4 pack4(float a, float b, float c, float d) {
-  bf4 r;
-  r[0] = (short)f2bf(a);
-  r[1] = (short)f2bf(b);
-  r[2] = (short)f2bf(c);
-  r[3] = (short)f2bf(d);
-  return r;
+  const bf16x2 lo = __builtin_convertvector(fl2{a, b}, bf16x2);
+  const bf16x2 hi = __builtin_convertvector(fl2{c, d}, bf16x2);
+  return __builtin_bit_cast(bf4, u32x2{__builtin_bit_cast(uint32_t, lo),
+                                       __builtin_bit_cast(uint32_t, hi)});
 }
 
 __device__ __forceinline__ fl4 mfma16(bf4 a, bf4 b, fl4 c) {
@@ -84,13 +89,18 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-/* sum over the 16 lanes of a row (lanes with equal l/16) */
+template <int CTRL> __device__ __forceinline__ float dpp_add(float v) {
+  return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                 0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+/* sum over the 16 lanes of a row (lanes with equal l/16) with DPP: quad swaps
+ * (quad_perm [1,0,3,2], [2,3,0,1]), then row rotations by 4 and 8 */
 __device__ __forceinline__ float row_sum16(float v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
-  v += __shfl_xor(v, 8, 64);
-  return v;
+  v = dpp_add<0xB1>(v);
+  v = dpp_add<0x4E>(v);
+  v = dpp_add<0x124>(v);
+  return dpp_add<0x128>(v);
 }
 
 /* ---- edge features -------------------------------------------------------- */
@@ -135,44 +145,114 @@ __global__ void __launch_bounds__(256) k_edge_features(DevCtx P, int inst, uint1
 
 /* ---- network kernels -------------------------------------------------------- */
 
-struct MlpLds {
+struct __align__(16) MlpLds {
   uint16_t w2[MLP_H * MLP_H];  /* W2 row-major [out][in] */
   uint16_t w2t[MLP_H * MLP_H]; /* W2 transposed [in][out] */
   uint16_t tr[16 * MLP_H];     /* transpose buffer of one 16-edge tile */
 };
 
+/* Lane l stages row l of W2 (64 BF16, 128 B) with eight independent 16-byte
+ * loads, so the wavefront waits for one memory latency, not 64. */
 __device__ __forceinline__ void stage_w2(MlpLds &L, const uint16_t *wb, int l) {
-  for (int i = l; i < MLP_H * MLP_H; i += 64) {
-    const uint16_t v = wb[BW2 + i];
-    L.w2[i] = v;
-    L.w2t[(i & 63) * MLP_H + (i >> 6)] = v;
+  const uint4 *src = reinterpret_cast<const uint4 *>(wb + BW2 + l * MLP_H);
+  uint4 v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = src[q];
+  uint4 *dst = reinterpret_cast<uint4 *>(L.w2 + l * MLP_H);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    dst[q] = v[q];
+    const uint32_t u[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+    for (int h = 0; h < 8; ++h)
+      L.w2t[(8 * q + h) * MLP_H + l] = (uint16_t)(u[h >> 1] >> (16 * (h & 1)));
   }
   wave_sync();
 }
 
-/* Forward pass of the 16 edges e0..e0+15 (rows past E are zero).  Returns
- * the pre-activations z1, z2 in D layout (edge 4g+r, neuron 16nt+c) and the
- * logits of edges 4g+r (identical in the 16 lanes of row g). */
-__device__ __forceinline__ void tile_forward(const uint16_t *__restrict__ X, long E, long e0,
-                                             const float *__restrict__ w,
-                                             const uint16_t *__restrict__ wb, MlpLds &L, int l,
-                                             fl4 (&z1)[4], fl4 (&z2)[4], fl4 &z3) {
+/* per-lane weights that stay in registers over all tiles of a wavefront */
+struct LaneW {
+  bf4 w1[4];                   /* B operand of layer 1: W1[16nt+c][4g..4g+3] (g < 2) */
+  float b1[4], b2[4], w3[4];   /* neuron 16nt+c */
+  float b3;
+};
+
+__device__ __forceinline__ void load_lane_w(LaneW &W, const float *__restrict__ w,
+                                            const uint16_t *__restrict__ wb, int l) {
   const int g = l >> 4, c = l & 15;
-  bf4 xa = {0, 0, 0, 0};
-  if (g < 2 && e0 + c < E) xa = *reinterpret_cast<const bf4 *>(X + (e0 + c) * MLP_F + 4 * g);
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-    bf4 wv = {0, 0, 0, 0};
-    if (g < 2) wv = *reinterpret_cast<const bf4 *>(wb + BW1 + (16 * nt + c) * MLP_F + 4 * g);
-    const float b = w[PB1 + 16 * nt + c];
+    W.w1[nt] = bf4{0, 0, 0, 0};
+    if (g < 2) W.w1[nt] = *reinterpret_cast<const bf4 *>(wb + BW1 + (16 * nt + c) * MLP_F + 4 * g);
+    W.b1[nt] = w[PB1 + 16 * nt + c];
+    W.b2[nt] = w[PB2 + 16 * nt + c];
+    W.w3[nt] = w[PW3 + 16 * nt + c];
+  }
+  W.b3 = w[PB3];
+}
+
+/* Tile inputs are loaded from clamped indices (always inside [0, E)) with no
+ * branch, and masked where they are used: a load issued for the next tile then
+ * stays in flight while the current tile computes. */
+struct TileIn {
+  bf4 xa;      /* lane (g < 2, c): features 4g..4g+3 of edge e0+c */
+  bf4 xb;      /* lane (g, c < 8): feature c of edges e0+4g..e0+4g+3 */
+  uint32_t y[4]; /* lane (g, .): labels of edges e0+4g..e0+4g+3 */
+};
+
+__device__ __forceinline__ long clamp_e(long e, long E) { return e < E ? e : E - 1; }
+
+__device__ __forceinline__ TileIn load_tile(const uint16_t *__restrict__ X,
+                                            const uint8_t *__restrict__ Y, long E, long e0,
+                                            int l) {
+  const int g = l >> 4, c = l & 15;
+  TileIn t;
+  t.xa = *reinterpret_cast<const bf4 *>(X + clamp_e(e0 + c, E) * MLP_F + 4 * (g & 1));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) t.xb[i] = (short)X[clamp_e(e0 + 4 * g + i, E) * MLP_F + (c & 7)];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) t.y[r] = Y ? (uint32_t)Y[clamp_e(e0 + 4 * g + r, E)] : 0u;
+  return t;
+}
+
+/* A operand of layer 1 (zero for g >= 2 and past E) */
+__device__ __forceinline__ bf4 tile_xa(const TileIn &t, long E, long e0, int l) {
+  const int g = l >> 4, c = l & 15;
+  const bool ok = g < 2 && e0 + c < E;
+  bf4 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) r[i] = ok ? t.xa[i] : (short)0;
+  return r;
+}
+
+/* B operand of dW1 = dZ1^T X (zero for c >= 8 and past E) */
+__device__ __forceinline__ bf4 tile_xb(const TileIn &t, long E, long e0, int l) {
+  const int g = l >> 4, c = l & 15;
+  bf4 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) r[i] = (c < MLP_F && e0 + 4 * g + i < E) ? t.xb[i] : (short)0;
+  return r;
+}
+
+/* Forward pass of one 16-edge tile (A operand xa; rows past E are zero).
+ * Returns the pre-activations z1, z2 in D layout (edge 4g+r, neuron 16nt+c)
+ * and the logits of edges 4g+r (identical in the 16 lanes of row g). */
+__device__ __forceinline__ void tile_forward(const LaneW &W, bf4 xa, MlpLds &L, int l,
+                                             fl4 (&z1)[4], bf4 (&h1b)[4], fl4 (&z2)[4],
+                                             fl4 &z3) {
+  const int g = l >> 4, c = l & 15;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const float b = W.b1[nt];
     fl4 acc = {b, b, b, b};
-    z1[nt] = mfma16(xa, wv, acc);
+    z1[nt] = mfma16(xa, W.w1[nt], acc);
+    h1b[nt] = pack4(fmaxf(z1[nt][0], 0.0f), fmaxf(z1[nt][1], 0.0f), fmaxf(z1[nt][2], 0.0f),
+                    fmaxf(z1[nt][3], 0.0f));
   }
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      L.tr[(4 * g + r) * MLP_H + 16 * nt + c] = f2bf(fmaxf(z1[nt][r], 0.0f));
+    for (int r = 0; r < 4; ++r) L.tr[(4 * g + r) * MLP_H + 16 * nt + c] = (uint16_t)h1b[nt][r];
   wave_sync();
   bf4 ha[4];
 #pragma unroll
@@ -181,7 +261,7 @@ __device__ __forceinline__ void tile_forward(const uint16_t *__restrict__ X, lon
   wave_sync();
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-    const float b = w[PB2 + 16 * nt + c];
+    const float b = W.b2[nt];
     fl4 acc = {b, b, b, b};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
@@ -191,14 +271,11 @@ __device__ __forceinline__ void tile_forward(const uint16_t *__restrict__ X, lon
   }
   fl4 s = {0, 0, 0, 0};
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    const float w3n = w[PW3 + 16 * nt + c];
+  for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) s[r] += fmaxf(z2[nt][r], 0.0f) * w3n;
-  }
-  const float b3 = w[PB3];
+    for (int r = 0; r < 4; ++r) s[r] = fmaf(fmaxf(z2[nt][r], 0.0f), W.w3[nt], s[r]);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) z3[r] = row_sum16(s[r]) + b3;
+  for (int r = 0; r < 4; ++r) z3[r] = row_sum16(s[r]) + W.b3;
 }
 
 /* logits of E edges; grid-stride over 16-edge tiles, one wavefront each */
@@ -208,27 +285,38 @@ __global__ void __launch_bounds__(64) k_mlp_forward(const uint16_t *__restrict__
                                                     float *__restrict__ logits) {
   __shared__ MlpLds L;
   const int l = threadIdx.x, g = l >> 4, c = l & 15;
+  LaneW W;
+  load_lane_w(W, w, wb, l);
   stage_w2(L, wb, l);
-  for (long tile = blockIdx.x; tile * 16 < E; tile += gridDim.x) {
+  long tile = blockIdx.x;
+  TileIn in = load_tile(X, nullptr, E, tile * 16, l);
+  for (; tile * 16 < E; tile += gridDim.x) {
     const long e0 = tile * 16;
+    const TileIn next = load_tile(X, nullptr, E, (tile + gridDim.x) * 16, l);
     fl4 z1[4], z2[4], z3;
-    tile_forward(X, E, e0, w, wb, L, l, z1, z2, z3);
+    bf4 h1b[4];
+    tile_forward(W, tile_xa(in, E, e0, l), L, l, z1, h1b, z2, z3);
     if (c == 0)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         if (e0 + 4 * g + r < E) logits[e0 + 4 * g + r] = z3[r];
+    in = next;
   }
 }
 
-/* Gradient of sum_e BCE(sigmoid(z_e), y_e) * inv_e and the BCE sum, one row
- * of PROW partial sums per wavefront (row = blockIdx.x). */
+/* Gradient of sum_e BCE(sigmoid(z_e), y_e) * inv_e and, when want_loss, the
+ * BCE sum (zero otherwise), one row of PROW partial sums per wavefront
+ * (row = blockIdx.x).  The next tile's features and labels are loaded while
+ * the current tile computes. */
 __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
                                                  const uint8_t *__restrict__ Y, long E,
                                                  float inv_e, const float *__restrict__ w,
                                                  const uint16_t *__restrict__ wb,
-                                                 float *__restrict__ part) {
+                                                 float *__restrict__ part, int want_loss) {
   __shared__ MlpLds L;
   const int l = threadIdx.x, g = l >> 4, c = l & 15;
+  LaneW W;
+  load_lane_w(W, w, wb, l);
   stage_w2(L, wb, l);
   fl4 acc1[4], acc2[4][4];
 #pragma unroll
@@ -239,39 +327,40 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
   }
   float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gw3[4] = {0, 0, 0, 0};
   float gb3 = 0.0f, loss = 0.0f;
-  for (long tile = blockIdx.x; tile * 16 < E; tile += gridDim.x) {
+  long tile = blockIdx.x;
+  TileIn in = load_tile(X, Y, E, tile * 16, l);
+  for (; tile * 16 < E; tile += gridDim.x) {
     const long e0 = tile * 16;
+    const TileIn next = load_tile(X, Y, E, (tile + gridDim.x) * 16, l);
     fl4 z1[4], z2[4], z3;
-    tile_forward(X, E, e0, w, wb, L, l, z1, z2, z3);
+    bf4 h1b[4];
+    tile_forward(W, tile_xa(in, E, e0, l), L, l, z1, h1b, z2, z3);
     fl4 dz3;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const long e = e0 + 4 * g + r;
       dz3[r] = 0.0f;
-      if (e < E) {
-        const float z = z3[r], y = (float)Y[e];
-        const float p = 1.0f / (1.0f + expf(-z));
+      if (e0 + 4 * g + r < E) {
+        const float z = z3[r], y = (float)in.y[r];
+        const float p = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
         dz3[r] = (p - y) * inv_e;
-        if (c == 0) loss += fmaxf(z, 0.0f) - z * y + log1pf(expf(-fabsf(z)));
+        if (want_loss && c == 0) loss += fmaxf(z, 0.0f) - z * y + log1pf(expf(-fabsf(z)));
       }
     }
     if (c == 0) gb3 += (dz3[0] + dz3[1]) + (dz3[2] + dz3[3]);
     fl4 dz2[4];
-    bf4 h1b[4], dz2b[4];
+    bf4 dz2b[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-      const float w3n = w[PW3 + 16 * nt + c];
+      const float w3n = W.w3[nt];
       float sw = 0.0f, sb = 0.0f;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        sw += dz3[r] * fmaxf(z2[nt][r], 0.0f);
+        sw = fmaf(dz3[r], fmaxf(z2[nt][r], 0.0f), sw);
         dz2[nt][r] = z2[nt][r] > 0.0f ? dz3[r] * w3n : 0.0f;
         sb += dz2[nt][r];
       }
       gw3[nt] += sw;
       gb2[nt] += sb;
-      h1b[nt] = pack4(fmaxf(z1[nt][0], 0.0f), fmaxf(z1[nt][1], 0.0f), fmaxf(z1[nt][2], 0.0f),
-                      fmaxf(z1[nt][3], 0.0f));
       dz2b[nt] = pack4(dz2[nt][0], dz2[nt][1], dz2[nt][2], dz2[nt][3]);
     }
     /* dW2 += dZ2^T H1 (K = the 16 edges) */
@@ -285,18 +374,12 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
 #pragma unroll
       for (int r = 0; r < 4; ++r) L.tr[(4 * g + r) * MLP_H + 16 * nt + c] = (uint16_t)dz2b[nt][r];
     wave_sync();
+    const bf4 xb = tile_xb(in, E, e0, l);
     bf4 da[4];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
       da[ks] = *reinterpret_cast<const bf4 *>(&L.tr[c * MLP_H + 16 * ks + 4 * g]);
     wave_sync();
-    bf4 xb = {0, 0, 0, 0};
-    if (c < MLP_F)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const long e = e0 + 4 * g + i;
-        xb[i] = e < E ? (short)X[e * MLP_F + c] : (short)0;
-      }
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
       fl4 dh = {0, 0, 0, 0};
@@ -316,6 +399,7 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
       /* dW1 += dZ1^T X (K = the 16 edges; feature columns 8..15 are zero) */
       acc1[nt] = mfma16(pack4(dz1[0], dz1[1], dz1[2], dz1[3]), xb, acc1[nt]);
     }
+    in = next;
   }
   float *row = part + (size_t)blockIdx.x * PROW;
 #pragma unroll
@@ -353,10 +437,26 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
   }
 }
 
-/* Sum the gradient rows in row order (deterministic); optionally write the
- * sums, and with lr > 0 take one Adam step (bias-corrected rate lr_t) and
- * refresh the BF16 matrix copies. */
-__global__ void __launch_bounds__(256) k_mlp_reduce(const float *__restrict__ part, int rows,
+/* First stage of the row reduction: split y (blockIdx.y) of RED_SPLIT sums
+ * rows [y*rows/RED_SPLIT, (y+1)*rows/RED_SPLIT) in row order into row y of
+ * part2.  The loads of a split are independent, so a thread waits for about
+ * one memory latency. */
+__global__ void __launch_bounds__(256) k_mlp_reduce_rows(const float *__restrict__ part, int rows,
+                                                         float *__restrict__ part2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= PROW) return;
+  const int y = blockIdx.y;
+  const int r0 = (int)((long)y * rows / RED_SPLIT), r1 = (int)((long)(y + 1) * rows / RED_SPLIT);
+  float s = 0.0f;
+#pragma unroll 8
+  for (int r = r0; r < r1; ++r) s += part[(size_t)r * PROW + p];
+  part2[(size_t)y * PROW + p] = s;
+}
+
+/* Sum the RED_SPLIT first-stage rows in order (deterministic); optionally
+ * write the sums, and with lr > 0 take one Adam step (bias-corrected rate
+ * lr_t) and refresh the BF16 matrix copies. */
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float *__restrict__ part2,
                                                     float *__restrict__ gsum, float *__restrict__ w,
                                                     float *__restrict__ m, float *__restrict__ v,
                                                     uint16_t *__restrict__ wb, float lr_t,
@@ -364,7 +464,8 @@ __global__ void __launch_bounds__(256) k_mlp_reduce(const float *__restrict__ pa
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= PROW) return;
   float s = 0.0f;
-  for (int r = 0; r < rows; ++r) s += part[(size_t)r * PROW + p];
+#pragma unroll
+  for (int r = 0; r < RED_SPLIT; ++r) s += part2[(size_t)r * PROW + p];
   if (gsum) gsum[p] = s;
   if (p >= NPAR || !(lr_t > 0.0f)) return;
   const float mm = beta1 * m[p] + (1.0f - beta1) * s;
@@ -396,6 +497,7 @@ struct AcoHipPrior {
   hipStream_t stream = nullptr;
   char err[512] = {0};
   float *d_w = nullptr, *d_m = nullptr, *d_v = nullptr, *d_part = nullptr, *d_gsum = nullptr;
+  float *d_part2 = nullptr; /* first-stage sums of the gradient rows */
   uint16_t *d_wb = nullptr;
   uint16_t *d_X = nullptr; /* training set features, E x 8 BF16 */
   uint8_t *d_y = nullptr;  /* training set labels */
@@ -460,6 +562,7 @@ extern "C" int aco_hip_prior_create(AcoHipPrior **out, int device, uint32_t seed
       hipMalloc((void **)&pr->d_v, sizeof(float) * NPAR) != hipSuccess ||
       hipMalloc((void **)&pr->d_gsum, sizeof(float) * PROW) != hipSuccess ||
       hipMalloc((void **)&pr->d_part, sizeof(float) * PROW * MLP_GRID) != hipSuccess ||
+      hipMalloc((void **)&pr->d_part2, sizeof(float) * PROW * RED_SPLIT) != hipSuccess ||
       hipMalloc((void **)&pr->d_wb, sizeof(uint16_t) * NBW) != hipSuccess)
     rc = ACO_HIP_ERR_ALLOCATION;
   if (rc == ACO_HIP_OK) {
@@ -490,7 +593,7 @@ extern "C" void aco_hip_prior_destroy(AcoHipPrior *pr) {
   if (!pr) return;
   (void)hipSetDevice(pr->device);
   if (pr->stream) (void)hipStreamSynchronize(pr->stream);
-  void *ptrs[] = {pr->d_w, pr->d_m, pr->d_v, pr->d_part, pr->d_gsum, pr->d_wb,
+  void *ptrs[] = {pr->d_w, pr->d_m, pr->d_v, pr->d_part, pr->d_part2, pr->d_gsum, pr->d_wb,
                   pr->d_X, pr->d_y, pr->d_tx, pr->d_tl, pr->d_ty};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -550,15 +653,18 @@ static int prior_launch_forward(AcoHipPrior *pr, hipStream_t st, const uint16_t 
 
 /* gradient rows for (X, Y, E), reduced into d_gsum; lr_t > 0 also steps */
 static int prior_launch_grad(AcoHipPrior *pr, const uint16_t *X, const uint8_t *Y, int64_t E,
-                             float lr_t) {
+                             float lr_t, int want_loss) {
   const long tiles = (long)((E + 15) / 16);
   const int rows = (int)std::max<long>(1, std::min<long>(tiles, MLP_GRID));
   hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(64), 0, pr->stream, X, Y, (long)E,
-                     1.0f / (float)E, pr->d_w, pr->d_wb, pr->d_part);
+                     1.0f / (float)E, pr->d_w, pr->d_wb, pr->d_part, want_loss);
   PLAUNCHED(pr, "k_mlp_grad");
+  hipLaunchKernelGGL(k_mlp_reduce_rows, dim3((PROW + 255) / 256, RED_SPLIT), dim3(256), 0,
+                     pr->stream, pr->d_part, rows, pr->d_part2);
+  PLAUNCHED(pr, "k_mlp_reduce_rows");
   hipLaunchKernelGGL(k_mlp_reduce, dim3((PROW + 255) / 256), dim3(256), 0, pr->stream,
-                     pr->d_part, rows, pr->d_gsum, pr->d_w, pr->d_m, pr->d_v, pr->d_wb, lr_t,
-                     0.9f, 0.999f, 1e-8f);
+                     pr->d_part2, pr->d_gsum, pr->d_w, pr->d_m, pr->d_v, pr->d_wb, lr_t, 0.9f,
+                     0.999f, 1e-8f);
   PLAUNCHED(pr, "k_mlp_reduce");
   return ACO_HIP_OK;
 }
@@ -588,7 +694,7 @@ extern "C" int aco_hip_prior_gradient(AcoHipPrior *pr, const uint16_t *features,
   PHIP(pr, hipMemcpyAsync(pr->d_tx, features, sizeof(uint16_t) * MLP_F * count,
                           hipMemcpyHostToDevice, pr->stream));
   PHIP(pr, hipMemcpyAsync(pr->d_ty, labels, count, hipMemcpyHostToDevice, pr->stream));
-  if ((rc = prior_launch_grad(pr, pr->d_tx, pr->d_ty, count, 0.0f))) return rc;
+  if ((rc = prior_launch_grad(pr, pr->d_tx, pr->d_ty, count, 0.0f, 1))) return rc;
   std::vector<float> g(PROW);
   PHIP(pr, hipMemcpyAsync(g.data(), pr->d_gsum, sizeof(float) * PROW, hipMemcpyDeviceToHost,
                           pr->stream));
@@ -696,7 +802,7 @@ extern "C" int aco_hip_prior_train(AcoHipPrior *pr, int steps, float lr, float *
     ++pr->step;
     const double b1t = 1.0 - pow(0.9, pr->step), b2t = 1.0 - pow(0.999, pr->step);
     const float lr_t = (float)(lr * sqrt(b2t) / b1t);
-    if ((rc = prior_launch_grad(pr, pr->d_X, pr->d_y, pr->E, lr_t))) return rc;
+    if ((rc = prior_launch_grad(pr, pr->d_X, pr->d_y, pr->E, lr_t, s == steps - 1))) return rc;
   }
   if (loss_out) {
     float l = 0.0f;
