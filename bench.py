@@ -152,7 +152,7 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
             "mfma_peak_tflops_bf16": 2500.0}
 
 
-def batched_leg(epochs: int, count: int = 512, ants: int = 128):
+def batched_leg(epochs: int, count: int = 512, ants: int = 128, fallback_matrix: int = -1):
     """C4: `count` instances in one context (n ~ U[50, 2000], half uniform,
     half clustered with sigma 30, K = ceil(n/125), demands U[1,10],
     Q = ceil(1.1 sum/K)), `ants` ants each, per-instance seeds, candidate
@@ -169,7 +169,7 @@ def batched_leg(epochs: int, count: int = 512, ants: int = 128):
         k_table = {int(c): k for c, k in _json.load(open(table_path))["k_by_size_class"].items()}
     ks = tuner.candidate_lengths(insts, k_table, default=16)
     t0 = time.perf_counter()
-    s = HipSolver(insts, Params(cand_k=16, ants=ants, rho=0.1),
+    s = HipSolver(insts, Params(cand_k=16, ants=ants, rho=0.1, fallback_matrix=fallback_matrix),
                   seeds=[9000 + i for i in range(count)], cand_k=ks)
     s.prepare()
     s.sync()
@@ -455,7 +455,10 @@ def main():
             "decisions_per_tour": round(dev["decisions"] / max(1, dev["ants"]), 1),
             "fallback_rate": round(dev["fallbacks"] / max(1, dev["decisions"]), 4),
             "fallback_cycle_share": round(dev["fallback_cycles"] / max(1, dev["ant_cycles"]), 4),
-            "cycles_per_decision": round(dev["ant_cycles"] / max(1, dev["decisions"]), 1)},
+            "cycles_per_decision": round(dev["ant_cycles"] / max(1, dev["decisions"]), 1),
+            "cycles_per_ant": round(dev["ant_cycles"] / max(1, dev["ants"])),
+            "staging_cycles_per_group": round(dev["staging_cycles"] * s.buckets()[0]["ants_per_group"]
+                                              / max(1, dev["ants"]))},
         "update_phases_us": ({
             k: round(1e6 * dev[k + "_ticks"] / max(1, dev["wall_clock_hz"]) / dev["updates"], 2)
             for k in ("evap", "select", "deposit")} if dev["updates"] else None),

@@ -95,9 +95,9 @@ static int fs_stride_of(int n1) { return (n1 + 15) & ~15; }
  * of k_construct): staged candidate rows (+64 padding entries), depot scores
  * and coordinates; the computed-fallback tables (taupow, and eta unless
  * eta_len == 0) when the weight matrix is off; per wave a state array of S
- * u16 and a tour buffer of `stride` u16. */
+ * u16, a tour buffer of `stride` u16 and a 128-byte store sink. */
 static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on, int stride) {
-  size_t b = (size_t)W * ((size_t)fs_stride_of(n1) * 2 + r16((size_t)stride * 2));
+  size_t b = (size_t)W * ((size_t)fs_stride_of(n1) * 2 + r16((size_t)stride * 2) + 128);
   if (!fs_on) b += 256 * 4 + r16((size_t)eta_len * 4);
   if (stage) {
     const size_t ent = (size_t)n1 * k + 64;
@@ -445,7 +445,11 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if (c->sd_eta) sd += r16((size_t)c->eta_len * 4);
   c->sd_lds = (sd <= 131072 && c->max_stride <= 65536) ? sd : 0;
   if (env_int("ACO_HIP_FUSED_UPDATE", 0)) c->sd_lds = 0; /* force the fused update kernel */
-  c->sd_blocks = std::max(1, std::min(64, (2 * c->max_stride + 255) / 256));
+  /* workgroups per colony: enough for 256 pair events each, but about 2048 in
+   * all (a batch of many colonies fills the chip with one or a few each, and
+   * every workgroup stages both tours) */
+  c->sd_blocks = std::max(
+      1, std::min({64, (2 * c->max_stride + 255) / 256, (2048 + c->num_col - 1) / c->num_col}));
   c->W = c->buckets[0].W;
   c->stage = c->buckets[0].stage;
   c->eta_lds = c->buckets[0].eta_lds;

@@ -418,6 +418,10 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   off += (size_t)W * Sn * 2;
   /* tour buffer of this wave: entry p of the tour (u16 node ids) */
   uint16_t *s_tour = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * round16((size_t)D.stride * 2));
+  off += (size_t)W * round16((size_t)D.stride * 2);
+  /* 64 u16 per wave that lanes 1..63 write instead of the state and tour
+   * entries lane 0 writes, so those stores need no exec mask */
+  uint16_t *s_sink = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * 128);
   __syncthreads();
   if (STATS && threadIdx.x == 0)
     atomicAdd(P.stats + (size_t)colony * ACO_STATS + 5, clock64() - t_kernel);
@@ -450,6 +454,10 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   float rf0 = 0.0f, rf1 = 0.0f; /* lane l: uniform draws of decision (s & ~63) + l */
   int pos = 0;
   if (lane == 0) s_tour[0] = 0;
+  /* lane 0 stores to state[v] and tour[p + 1], the others to their sink */
+  const int wscale = lane == 0 ? 1 : 0;
+  uint16_t *const st_w = lane == 0 ? s_state : s_sink + lane;
+  uint16_t *const tr_w = lane == 0 ? s_tour + 1 : s_sink + lane;
 
   for (int veh = 0; veh < K; ++veh) {
     int cur = 0;
@@ -513,38 +521,38 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
       const int j2 = (int)c_idx[e2];
       const float sc2 = c_score[e2];
       const float dsc2 = c_dsc[mv];
-      /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand;
-       * depot scores are finite or 0 (never NaN), so dsc > 0 is a test of
-       * the sign-and-magnitude bits.  A fallback that found nothing (move
-       * -1) closes the route whatever this roulette draws. */
-      if (rem_dem <= close_cap) {
-        const int dbits = __builtin_amdgcn_readfirstlane(__float_as_int(dsc_cur));
-        if (dbits > 0) {
-          const float thr2 = f_readlane(rf1, sl) * (T + __int_as_float(dbits));
-          if (thr2 >= T) move = 0;
-        }
+      /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand,
+       * without branches: dsc_cur is the same in every lane; depot scores
+       * are finite or 0 (never NaN), so dsc > 0 is a test of the
+       * sign-and-magnitude bits.  A fallback that found nothing (move -1)
+       * closes the route whatever this roulette draws. */
+      {
+        const float thr2 = f_readlane(rf1, sl) * (T + dsc_cur);
+        const uint64_t cm = ballot64(__float_as_int(dsc_cur) > 0) & ballot64(thr2 >= T);
+        if (rem_dem <= close_cap && cm != 0) move = 0;
       }
-      if (move > 0) {
-        const uint16_t st2 = s_state[j2]; /* read before state[move] changes: j2 != move */
-        ++pos;
-        if (lane == 0) {
-          s_state[move] = (uint16_t)VISITED;
-          s_tour[pos] = (uint16_t)move;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        load += mdem;
-        rem_dem -= mdem;
-        --rem_cnt;
-        cur = move;
-        j = j2;
-        sc = sc2;
-        dsc_cur = dsc2;
-        st = st2;
-      } else {
-        cur = -1; /* nothing fits (fallback found none) or the depot closes */
-      }
+      /* the move's bookkeeping, also without branches: with no move (a close,
+       * or a fallback that found nothing) lane 0 marks the depot visited
+       * again and writes tour[pos + 1] = 0, which the route close writes
+       * anyway; the loop then ends on cur = -1 */
+      const bool moved = move > 0;
+      const int mvw = moved ? move : 0;
+      const uint16_t st2 = s_state[j2]; /* read before state[move] changes: j2 != move */
+      st_w[mvw * wscale] = (uint16_t)VISITED;
+      tr_w[pos * wscale] = (uint16_t)mvw;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int md = moved ? mdem : 0;
+      pos += moved ? 1 : 0;
+      load += md;
+      rem_dem -= md;
+      rem_cnt -= moved ? 1 : 0;
+      cur = moved ? move : -1;
+      j = j2;
+      sc = sc2;
+      dsc_cur = dsc2;
+      st = st2;
     }
     /* route close (aco_cuda_kernels.cu:547-552) */
     ++pos;
