@@ -850,9 +850,12 @@ static int launch_evap_flat(AcoHipContext *c) {
   const long long blocks = std::max(1LL, std::min((c->flat_total + 255) / 256, 16384LL));
   {
     Timed tm(c, T_EVAP);
-    /* the taupow table, and the eta table while it is at most 32 KB, in LDS */
+    /* the colony offsets (up to EVAP_LDS_COLONIES colonies), the taupow
+     * table, and the eta table while it is at most 32 KB, in LDS */
     const int eta_lds = c->eta_len <= 8192 ? 1 : 0;
-    const size_t lds = 256 * 4 + (eta_lds ? (size_t)c->eta_len * 4 : 0);
+    const size_t off_lds =
+        c->num_col <= EVAP_LDS_COLONIES ? 2 * sizeof(long long) * ((size_t)c->num_col + 1) : 0;
+    const size_t lds = off_lds + 256 * 4 + (eta_lds ? (size_t)c->eta_len * 4 : 0);
     hipLaunchKernelGGL(k_evaporate_flat, dim3((unsigned)blocks), dim3(256), lds, c->stream, c->P,
                        c->d_item_off, c->d_slot_off, c->num_col, eta_lds);
   }

@@ -109,6 +109,8 @@ struct DevCtx {
  * ticks, [9] deposit ticks (wall_clock64), [10] ~(first block start) of the
  * running update (reset to 0 by its last block) */
 #define ACO_STATS 16
+/* colonies whose evaporation offsets k_evaporate_flat stages in LDS */
+#define EVAP_LDS_COLONIES 1024
 
 #define ACO_HIST 64
 struct HistEntry {

@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
 
   for (int veh = 0; veh < K; ++veh) {
     int cur = 0;
-    int load = 0;
+    int room = Q; /* Q - load */
     const int close_cap = (K - veh - 1) * Q; /* close allowed iff rem_dem <= close_cap */
     /* the current node's row: lane t holds candidate t (index, score, state)
      * and every lane the node's depot score.  After a move the next row is
@@ -475,8 +475,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
     uint16_t st = s_state[j]; /* 16-bit loop variable: widened at its use */
     /* aco_cuda_kernels.cu:492: the route goes on while rem_cnt > 0 and
      * load < Q; cur = -1 marks a route closed by a roulette (one exit test) */
-    while (((rem_cnt - 1) | (Q - 1 - load) | cur) >= 0) {
-      const int room = Q - load;
+    while (((rem_cnt - 1) | (room - 1) | cur) >= 0) {
       const int sl = (int)(s & 63u);
       if (sl == 0) {
         uint32_t r[4];
@@ -545,7 +544,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const int md = moved ? mdem : 0;
       pos += moved ? 1 : 0;
-      load += md;
+      room -= md;
       rem_dem -= md;
       rem_cnt -= moved ? 1 : 0;
       cur = moved ? move : -1;
@@ -748,33 +747,60 @@ __device__ __forceinline__ int colony_of(const long long *off, int n, long long 
 /* Evaporation of every colony of the context in one flat grid: items
  * [0, item_off[ncol]) are the colonies' matrix items (all-pairs weight items
  * of fs_item, or 16-byte pheromone vectors without the weight matrix),
- * items after them the candidate slots; each thread finds its colony by a
- * binary search of the prefix offsets.  A colony evaporates iff it is not
+ * items after them the candidate slots.  Each workgroup takes a contiguous
+ * range of items and its threads stride through it by the workgroup size, so
+ * a thread's consecutive items nearly always belong to one colony: the
+ * colony (a binary search of the prefix offsets, staged in LDS when there are
+ * at most EVAP_LDS_COLONIES) and its descriptors are looked up again only
+ * when an item leaves the cached range.  A colony evaporates iff it is not
  * halted and some ant of this epoch was feasible (P.feas, tagged with the
  * epoch by k_construct) - the condition of k_evaporate with from_ants. */
 __global__ void __launch_bounds__(256) k_evaporate_flat(DevCtx P, const long long *item_off,
                                                         const long long *slot_off, int ncol,
                                                         int eta_lds) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dyn_smem[];
-  float *s_tp = reinterpret_cast<float *>(dyn_smem);
+  long long *s_ioff = reinterpret_cast<long long *>(dyn_smem);
+  long long *s_soff = s_ioff + (ncol + 1);
+  const bool off_lds = ncol <= EVAP_LDS_COLONIES;
+  float *s_tp = reinterpret_cast<float *>(off_lds ? (unsigned char *)(s_soff + (ncol + 1))
+                                                  : dyn_smem);
   float *s_eta = s_tp + 256;
+  if (off_lds)
+    for (int i = threadIdx.x; i <= ncol; i += blockDim.x) {
+      s_ioff[i] = item_off[i];
+      s_soff[i] = slot_off[i];
+    }
   for (int i = threadIdx.x; i < 256; i += blockDim.x) s_tp[i] = P.taupow[i];
   if (eta_lds)
     for (int i = threadIdx.x; i < P.eta_len; i += blockDim.x) s_eta[i] = P.eta[i];
   __syncthreads();
+  const long long *ioff = off_lds ? (const long long *)s_ioff : item_off;
+  const long long *soff = off_lds ? (const long long *)s_soff : slot_off;
   const float *eta = eta_lds ? (const float *)s_eta : P.eta;
-  const long long ti = item_off[ncol], ts = slot_off[ncol];
+  const long long ti = ioff[ncol], total = ti + soff[ncol];
+  const long long per = ((total + gridDim.x - 1) / gridDim.x + 255) & ~255LL;
+  const long long g0 = (long long)blockIdx.x * per;
+  const long long g1 = g0 + per < total ? g0 + per : total;
   const uint32_t d = P.q_evap;
-  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < ti + ts;
-       g += (long long)gridDim.x * blockDim.x) {
-    const bool mat = g < ti;
-    const long long gg = mat ? g : g - ti;
-    const int c = colony_of(mat ? item_off : slot_off, ncol, gg);
-    const long long e = gg - (mat ? item_off[c] : slot_off[c]);
-    const AcoHipColonyState *S = P.state + c;
-    if (S->halted || P.feas[c] != S->epoch + 1) continue;
-    const ColonyDesc D = P.col[c];
-    const InstDesc I = P.inst[D.inst];
+  long long lo = 0, hi = 0; /* flat range of the cached colony */
+  bool mat = false, ok = false;
+  ColonyDesc D;
+  InstDesc I;
+  for (long long g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
+    if (g < lo || g >= hi) {
+      mat = g < ti;
+      const long long *off = mat ? ioff : soff;
+      const long long base = mat ? 0 : ti;
+      const int c = colony_of(off, ncol, g - base);
+      lo = base + off[c];
+      hi = base + off[c + 1];
+      const AcoHipColonyState *S = P.state + c;
+      ok = !S->halted && P.feas[c] == S->epoch + 1;
+      D = P.col[c];
+      I = P.inst[D.inst];
+    }
+    if (!ok) continue;
+    const long long e = g - lo;
     if (mat) {
       if (P.fscore) {
         fs_item(P, D, I, s_tp, eta, (uint32_t)e, true, d);
