@@ -259,7 +259,7 @@ def tour_is_feasible(inst, tour) -> bool:
 
 
 def quality_leg(seeds: int = 32, n: int = 100, ants: int = 64):
-    """Solution quality against the reference's sequential solver at equal
+    """CPU-baseline leg: solution quality against the reference's sequential solver at equal
     epochs: unit-demand uniform instances (the reference parser's format),
     n customers, K=8, Q=ceil(1.2 n/8); the reference sequential CLI (built
     from its sources) runs to its stagnation stop (50 epochs) and reports
@@ -327,7 +327,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 512-instance and n=5000 legs")
     ap.add_argument("--no-quality", action="store_true",
-                    help="skip the quality comparison with the reference sequential solver")
+                    help="skip the quality comparison with the reference sequential solver "
+                         "(part of the CPU baselines, so --no-cpu skips it too)")
     ap.add_argument("--no-prior", action="store_true",
                     help="skip the learned-prior vs plain ACO leg")
     ap.add_argument("--prior-seconds", type=float, default=3.0,
@@ -467,15 +468,17 @@ def main():
     if not args.no_batch:
         out["batched"] = batched_leg(20)
         out["large_clustered"] = large_clustered_leg(20)
-    if not args.no_quality:
-        out["quality_vs_reference_seq"] = quality_leg(32)
     if not args.no_prior:
         out["learned_prior"] = learned_prior_leg(args.prior_seconds, 4, 64)
     if not args.no_cpu:
+        # the CPU baselines: the port, the reference's OpenMP binary, and the
+        # reference's sequential solver as the quality baseline at equal epochs
         out["cpu_baseline"] = cpu_baseline(inst)
         ref = reference_openmp_baseline()
         if ref:
             out["cpu_baseline"]["reference_openmp"] = ref
+        if not args.no_quality:
+            out["cpu_baseline"]["quality_vs_reference_seq"] = quality_leg(32)
     print(json.dumps(out))
 
 
