@@ -56,17 +56,20 @@ def cpu_baseline(inst, seconds_budget=12.0):
                       f"({dt:.1f} s), oracle/aco_oracle.c with OpenMP over ants"}
 
 
-def reference_openmp_baseline(seconds_budget=8.0):
+def reference_openmp_baseline(seconds_budget=8.0, width20=False):
     """The reference OpenMP CLI (built from its sources by oracle/ref.mk) on a
     unit-demand n=1000, K=8 instance of the same generator (the reference
     parser rejects weighted demand), 256 ants, ACO_SOLVER_FIXED_EPOCHS; its own
-    "Time:" line (epoch loop only).  Reported beside the port; None when the
-    binary was not built."""
+    "Time:" line (epoch loop only).  Its candidate width comes from the host's
+    L3 size (k = 0.7 L3 / ((n+1) 8), clamped to [16, 512]); width20 runs the
+    same solver with the L3 probe answered "224K" (oracle/ref_omp_l3_driver.c),
+    which gives k = 20, this solver's width.  Reported beside the port; None
+    when the binary was not built."""
     import re
     import subprocess
     import tempfile
     from vehicleroutingproblem_amd import Instance
-    exe = os.path.join(ROOT, "oracle", "_ref", "aco_omp")
+    exe = os.path.join(ROOT, "oracle", "_ref", "aco_omp_l3" if width20 else "aco_omp")
     if not os.path.exists(exe):
         return None
     inst = Instance.uniform(N_CUST, K_VEH, side=1000, dmin=1, dmax=1, slack_pct=120, seed=1)
@@ -78,6 +81,8 @@ def reference_openmp_baseline(seconds_budget=8.0):
         def run(epochs):
             env = dict(os.environ, ACO_SOLVER_FIXED_EPOCHS=str(epochs),
                        ACO_SOLVER_PROGRESS_INTERVAL_SECONDS="0", OMP_NUM_THREADS=str(threads))
+            if width20:
+                env["REF_L3_SIZE"] = "224K"
             out = subprocess.run([exe, path, str(K_VEH), str(ANTS), "1"], env=env,
                                  capture_output=True, text=True, timeout=300).stdout
             m = re.search(r"Time: ([0-9.]+)s", out)
@@ -93,8 +98,10 @@ def reference_openmp_baseline(seconds_budget=8.0):
     return {"value": round(ANTS * epochs / t, 1), "unit": "ant-tours/s", "cores": threads,
             "kind": "reference",
             "sample": f"reference src/openmp-mpi CLI, {epochs} fixed epochs x {ANTS} ants, "
-                      f"unit-demand n=1000 K=8 (real distances, rho=0.5, L3-tuned "
-                      f"candidate width), {t:.2f} s solver time"}
+                      f"unit-demand n=1000 K=8 (real distances, rho=0.5, "
+                      + ("candidate width 20: L3 probe answered 224K" if width20 else
+                         "L3-tuned candidate width as shipped")
+                      + f"), {t:.2f} s solver time"}
 
 
 # algorithmic flops per edge and training step: forward 2(8*64 + 64*64 + 64),
@@ -477,6 +484,9 @@ def main():
         ref = reference_openmp_baseline()
         if ref:
             out["cpu_baseline"]["reference_openmp"] = ref
+        ref20 = reference_openmp_baseline(width20=True)
+        if ref20:
+            out["cpu_baseline"]["reference_openmp_width20"] = ref20
         if not args.no_quality:
             out["cpu_baseline"]["quality_vs_reference_seq"] = quality_leg(32)
     print(json.dumps(out))

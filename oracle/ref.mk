@@ -13,7 +13,7 @@ COMMON := $(REF)/src/common/aco_config.c $(REF)/src/common/aco_shared.c \
           $(REF)/src/common/cli_common.c $(REF)/src/common/instance_parser.c \
           $(REF)/src/common/matrix.c $(REF)/src/common/solution.c
 
-all: $(OUT)/aco_seq $(OUT)/aco_omp $(OUT)/ref_knn
+all: $(OUT)/aco_seq $(OUT)/aco_omp $(OUT)/aco_omp_l3 $(OUT)/ref_knn
 
 $(OUT):
 	mkdir -p $@
@@ -23,6 +23,11 @@ $(OUT)/aco_seq: | $(OUT)
 
 $(OUT)/aco_omp: | $(OUT)
 	$(CC) $(CFLAGS) -fopenmp -o $@ $(REF)/src/main.c $(REF)/src/openmp-mpi/aco_parallel.c $(COMMON) -lm
+
+# the OpenMP CLI with its L3-size probe settable (REF_L3_SIZE), to time it
+# at a chosen candidate width
+$(OUT)/aco_omp_l3: ref_omp_l3_driver.c | $(OUT)
+	$(CC) $(CFLAGS) -fopenmp -I$(REF)/src/openmp-mpi -o $@ $(REF)/src/main.c ref_omp_l3_driver.c $(COMMON) -lm
 
 $(OUT)/ref_knn: ref_knn_driver.c | $(OUT)
 	$(CC) $(CFLAGS) -fopenmp -I$(REF)/src/openmp-mpi -o $@ ref_knn_driver.c $(COMMON) -lm

@@ -10,6 +10,7 @@ import pytest
 
 from vehicleroutingproblem_amd import HipSolver, Instance, Params
 from vehicleroutingproblem_amd import tuner
+import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -85,3 +86,28 @@ def test_bucket_shape_requests(gpu):
     with pytest.raises(Exception):
         s.set_bucket_shape(0, 9, 1)
     s.close()
+
+
+def test_many_colonies_match_oracle(gpu):
+    """More colonies than the flat evaporation stages offsets for in LDS
+    (1040 > 1024): 520 small instances, two colonies each.  Sampled colonies
+    equal the oracle run of that instance and colony alone (pheromone,
+    candidate-aligned copy, best cost) after three epochs, so the update
+    kernels address every colony's range correctly on that path too."""
+    rng = np.random.default_rng(5)
+    insts = [Instance.uniform(int(rng.integers(5, 31)), 2, seed=600 + i) for i in range(520)]
+    seeds = [7000 + i for i in range(520)]
+    p = Params(cand_k=8, ants=8, colonies=2, rho=0.5)
+    b = HipSolver(insts, p, seeds=seeds)
+    assert b.num_colonies == 1040
+    b.prepare()
+    b.epochs(3)
+    for i, c in ((0, 0), (1, 1), (257, 0), (518, 1), (519, 0), (519, 1)):
+        o = O.Oracle(insts[i], cand_k=8, ants=8, rho=0.5, seed=seeds[i], colony=c)
+        o.epochs(3)
+        tg, cg = b.tau(2 * i + c)
+        to, co = o.tau()
+        assert np.array_equal(tg, to), (i, c)
+        assert np.array_equal(cg, co), (i, c)
+        assert b.colony_state(2 * i + c)["global_best_cost"] == o.state()["global_best_cost"], (i, c)
+    b.close()
