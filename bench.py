@@ -316,6 +316,64 @@ def quality_leg(seeds: int = 32, n: int = 100, ants: int = 64):
             "mean_cost": round(sum(ours) / len(ours), 3)}
 
 
+def reference_campaign_leg(budget_s: float = 2.0):
+    """The reference's own benchmark instances n500_k8_s19000 and
+    n1000_k8_s19001 (copied in tests/golden) with the reference CUDA
+    campaign's settings (m=256, solver seeds 1234-1236, k=32), solved by the
+    campaign runner: all six runs in one batched context under one wall-clock
+    budget (ACO_SOLVER_TIMEOUT_SECONDS), against the best costs and per-run
+    elapsed times that campaign recorded
+    (tests/golden/ref_cuda_backend_sizes_n500_n1000.csv).  Costs are real
+    (unrounded) route lengths on both sides."""
+    import csv
+    import tempfile
+    from vehicleroutingproblem_amd import campaign
+    golden = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(golden, "ref_cuda_backend_sizes_n500_n1000.csv"), newline="") as f:
+        recorded = {(r["name"], int(r["run_id"])): r for r in csv.DictReader(f)}
+    keys = ("ACO_SOLVER_TIMEOUT_SECONDS", "ACO_SOLVER_STAGNATION_EPOCHS",
+            "ACO_SOLVER_FIXED_EPOCHS", "ACO_HIP_CAND_K")
+    saved = {k: os.environ.get(k) for k in keys}
+    try:
+        for k in keys:
+            os.environ.pop(k, None)
+        os.environ["ACO_SOLVER_TIMEOUT_SECONDS"] = str(budget_s)
+        with tempfile.TemporaryDirectory() as td:
+            manifest = os.path.join(td, "manifest.csv")
+            campaign.write_manifest(manifest, [
+                {"profile": "generated", "name": name, "instance_path": f"ref_{name}.vrp",
+                 "n": n, "K": 8, "m": 256, "solver_seed": 1234, "instance_seed": seed,
+                 "layout_id": "grid100", "capacity_formula": "ceil((1+20/100)*n/K)"}
+                for name, n, seed in (("n500_k8_s19000", 500, 19000),
+                                      ("n1000_k8_s19001", 1000, 19001))])
+            rows = campaign.solve(manifest, os.path.join(td, "runs.csv"), os.path.join(td, "sol"),
+                                  repeats=3, root=golden)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    runs, diffs = [], []
+    for r in rows:
+        ref = recorded[(r["name"], r["run_id"])]
+        ok = r["status"] == "ok"
+        if ok:
+            diffs.append((float(r["best_cost"]) - float(ref["best_cost"])) / float(ref["best_cost"]))
+        runs.append({"name": r["name"], "seed": r["solver_seed"],
+                     "best_cost": float(r["best_cost"]) if ok else None,
+                     "reference_cuda_best_cost": float(ref["best_cost"]),
+                     "reference_cuda_elapsed_s": float(ref["elapsed_s"])})
+    return {"workload": "reference instances n500_k8_s19000 and n1000_k8_s19001 (unit demand, "
+                        "K=8), m=256, seeds 1234-1236, k=32, rho=0.5: the reference CUDA "
+                        "campaign's recorded runs against one batched solve of all six",
+            "budget_s_all_six_runs": budget_s,
+            "batch_elapsed_s": float(rows[0]["elapsed_s"]) if rows and rows[0]["elapsed_s"] else None,
+            "all_ok": all(r["status"] == "ok" for r in rows),
+            "mean_best_cost_diff_pct": round(100 * sum(diffs) / len(diffs), 3) if diffs else None,
+            "runs": runs}
+
+
 def div_chunk(epochs: int, cap: int) -> int:
     """Largest divisor of `epochs` up to `cap`: graph replays of that many
     epochs end exactly on the last epoch."""
@@ -336,6 +394,8 @@ def main():
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the quality comparison with the reference sequential solver "
                          "(part of the CPU baselines, so --no-cpu skips it too)")
+    ap.add_argument("--no-campaign", action="store_true",
+                    help="skip the reference-instance campaign leg")
     ap.add_argument("--no-prior", action="store_true",
                     help="skip the learned-prior vs plain ACO leg")
     ap.add_argument("--prior-seconds", type=float, default=3.0,
@@ -477,6 +537,8 @@ def main():
         out["large_clustered"] = large_clustered_leg(20)
     if not args.no_prior:
         out["learned_prior"] = learned_prior_leg(args.prior_seconds, 4, 64)
+    if not args.no_campaign:
+        out["reference_campaign"] = reference_campaign_leg(2.0)
     if not args.no_cpu:
         # the CPU baselines: the port, the reference's OpenMP binary, and the
         # reference's sequential solver as the quality baseline at equal epochs

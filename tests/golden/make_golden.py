@@ -8,6 +8,9 @@ oracle/ref.mk builds from its sources).  Fixtures written:
   ref_<run>.solution.txt    tracked reference solution files (routes + cost lines)
   ref_smoke_runs.json       routes/costs of the reference seq and OpenMP CLIs on
                             the smoke-test instance (tools/bash/smoke_test.sh:17-48)
+  ref_cuda_backend_sizes_n500_n1000.csv
+                            the reference CUDA campaign's result rows (data) for
+                            the two instances above, m=256, seeds 1234-1236
 usage: python tests/golden/make_golden.py
 """
 import json
@@ -81,6 +84,14 @@ def main():
         base = os.path.basename(rel).replace("_solution.txt", "")
         with open(os.path.join(HERE, f"ref_{base}.solution.txt"), "w") as f:
             f.write("\n".join(keep) + "\n")
+    rows = []
+    for n in (500, 1000):
+        lines = open(os.path.join(REF, "results/manual_campaign/backend_sizes/cuda", f"n{n}",
+                                  "csv/manifest_cuda_per_instance_results.csv")).read().splitlines()
+        header = lines[0]
+        rows += [l for l in lines[1:] if l.strip()]
+    with open(os.path.join(HERE, "ref_cuda_backend_sizes_n500_n1000.csv"), "w") as f:
+        f.write("\n".join([header] + rows) + "\n")
     smoke = os.path.join(HERE, "ref_smoke.vrp")
     with open(smoke, "w") as f:
         f.write(SMOKE)
