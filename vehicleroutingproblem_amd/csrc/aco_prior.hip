@@ -15,9 +15,11 @@
  * A: lane l holds A[l%16][4(l/16)+i], B: B[4(l/16)+i][l%16], D: D[4(l/16)+r][l%16].
  * The backward products dW2 = dZ2^T H1 and dW1 = dZ1^T X take both operands
  * straight from D-layout registers; H1 (layer 2) and dZ2 (dH1 = dZ2 W2) are
- * transposed through 2 KB of LDS.  Gradients are summed per wavefront in
- * registers over its tiles, written as one row per wavefront, and reduced in
- * a fixed order in two stages (deterministic) before the update.
+ * transposed through LDS (rows padded against bank conflicts, as are the LDS
+ * copies of W2 and W2^T).  Gradients are summed per wavefront in registers
+ * over its tiles, the wavefronts of a workgroup are added in LDS, one row is
+ * written per workgroup, and the rows are reduced in a fixed order in two
+ * stages (deterministic) before the update.
  */
 #include <hip/hip_runtime.h>
 
@@ -50,6 +52,11 @@
 
 #define MLP_GRID 1024 /* wavefronts of the gradient kernel (rows of partial sums) */
 #define RED_SPLIT 32  /* first-stage splits of the row reduction */
+#define GRAD_WAVES 2  /* wavefronts per gradient workgroup */
+/* LDS row stride (BF16 elements) of W2, W2^T and the transpose buffers: 68
+ * (34 dwords) puts the 16 rows a 16-lane group reads with one ds_read_b64 on
+ * 16 distinct bank pairs; a 64-element stride put them all on one bank. */
+#define MLP_HS 68
 
 typedef short bf4 __attribute__((ext_vector_type(4)));
 typedef float fl4 __attribute__((ext_vector_type(4)));
@@ -146,28 +153,31 @@ __global__ void __launch_bounds__(256) k_edge_features(DevCtx P, int inst, uint1
 /* ---- network kernels -------------------------------------------------------- */
 
 struct __align__(16) MlpLds {
-  uint16_t w2[MLP_H * MLP_H];  /* W2 row-major [out][in] */
-  uint16_t w2t[MLP_H * MLP_H]; /* W2 transposed [in][out] */
-  uint16_t tr[16 * MLP_H];     /* transpose buffer of one 16-edge tile */
+  uint16_t w2[MLP_H * MLP_HS];  /* W2 row-major [out][in], rows padded to MLP_HS */
+  uint16_t w2t[MLP_H * MLP_HS]; /* W2 transposed [in][out] */
+  uint16_t tr[16 * MLP_HS];     /* transpose buffer of one 16-edge tile */
 };
 
-/* Lane l stages row l of W2 (64 BF16, 128 B) with eight independent 16-byte
- * loads, so the wavefront waits for one memory latency, not 64. */
-__device__ __forceinline__ void stage_w2(MlpLds &L, const uint16_t *wb, int l) {
+/* Thread l < 64 stages row l of W2 (64 BF16, 128 B) with eight independent
+ * 16-byte loads, so a wavefront waits for one memory latency, not 64; the
+ * caller synchronises (wave_sync for one wavefront, __syncthreads for more). */
+__device__ __forceinline__ void stage_w2(uint16_t *w2, uint16_t *w2t, const uint16_t *wb,
+                                         int l) {
+  if (l >= MLP_H) return;
   const uint4 *src = reinterpret_cast<const uint4 *>(wb + BW2 + l * MLP_H);
   uint4 v[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[q] = src[q];
-  uint4 *dst = reinterpret_cast<uint4 *>(L.w2 + l * MLP_H);
+  uint2 *dst = reinterpret_cast<uint2 *>(w2 + l * MLP_HS); /* 8-byte aligned rows */
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    dst[q] = v[q];
+    dst[2 * q] = make_uint2(v[q].x, v[q].y);
+    dst[2 * q + 1] = make_uint2(v[q].z, v[q].w);
     const uint32_t u[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
 #pragma unroll
     for (int h = 0; h < 8; ++h)
-      L.w2t[(8 * q + h) * MLP_H + l] = (uint16_t)(u[h >> 1] >> (16 * (h & 1)));
+      w2t[(8 * q + h) * MLP_HS + l] = (uint16_t)(u[h >> 1] >> (16 * (h & 1)));
   }
-  wave_sync();
 }
 
 /* per-lane weights that stay in registers over all tiles of a wavefront */
@@ -237,9 +247,9 @@ __device__ __forceinline__ bf4 tile_xb(const TileIn &t, long E, long e0, int l) 
 /* Forward pass of one 16-edge tile (A operand xa; rows past E are zero).
  * Returns the pre-activations z1, z2 in D layout (edge 4g+r, neuron 16nt+c)
  * and the logits of edges 4g+r (identical in the 16 lanes of row g). */
-__device__ __forceinline__ void tile_forward(const LaneW &W, bf4 xa, MlpLds &L, int l,
-                                             fl4 (&z1)[4], bf4 (&h1b)[4], fl4 (&z2)[4],
-                                             fl4 &z3) {
+__device__ __forceinline__ void tile_forward(const LaneW &W, bf4 xa, const uint16_t *w2,
+                                             uint16_t *tr, int l, fl4 (&z1)[4], bf4 (&h1b)[4],
+                                             fl4 (&z2)[4], fl4 &z3) {
   const int g = l >> 4, c = l & 15;
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
@@ -252,12 +262,12 @@ __device__ __forceinline__ void tile_forward(const LaneW &W, bf4 xa, MlpLds &L, 
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) L.tr[(4 * g + r) * MLP_H + 16 * nt + c] = (uint16_t)h1b[nt][r];
+    for (int r = 0; r < 4; ++r) tr[(4 * g + r) * MLP_HS + 16 * nt + c] = (uint16_t)h1b[nt][r];
   wave_sync();
   bf4 ha[4];
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks)
-    ha[ks] = *reinterpret_cast<const bf4 *>(&L.tr[c * MLP_H + 16 * ks + 4 * g]);
+    ha[ks] = *reinterpret_cast<const bf4 *>(&tr[c * MLP_HS + 16 * ks + 4 * g]);
   wave_sync();
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
@@ -265,7 +275,7 @@ __device__ __forceinline__ void tile_forward(const LaneW &W, bf4 xa, MlpLds &L, 
     fl4 acc = {b, b, b, b};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
-      acc = mfma16(ha[ks], *reinterpret_cast<const bf4 *>(&L.w2[(16 * nt + c) * MLP_H + 16 * ks + 4 * g]),
+      acc = mfma16(ha[ks], *reinterpret_cast<const bf4 *>(&w2[(16 * nt + c) * MLP_HS + 16 * ks + 4 * g]),
                    acc);
     z2[nt] = acc;
   }
@@ -287,7 +297,8 @@ __global__ void __launch_bounds__(64) k_mlp_forward(const uint16_t *__restrict__
   const int l = threadIdx.x, g = l >> 4, c = l & 15;
   LaneW W;
   load_lane_w(W, w, wb, l);
-  stage_w2(L, wb, l);
+  stage_w2(L.w2, L.w2t, wb, l);
+  wave_sync();
   long tile = blockIdx.x;
   TileIn in = load_tile(X, nullptr, E, tile * 16, l);
   for (; tile * 16 < E; tile += gridDim.x) {
@@ -295,7 +306,7 @@ __global__ void __launch_bounds__(64) k_mlp_forward(const uint16_t *__restrict__
     const TileIn next = load_tile(X, nullptr, E, (tile + gridDim.x) * 16, l);
     fl4 z1[4], z2[4], z3;
     bf4 h1b[4];
-    tile_forward(W, tile_xa(in, E, e0, l), L, l, z1, h1b, z2, z3);
+    tile_forward(W, tile_xa(in, E, e0, l), L.w2, L.tr, l, z1, h1b, z2, z3);
     if (c == 0)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
@@ -304,20 +315,43 @@ __global__ void __launch_bounds__(64) k_mlp_forward(const uint16_t *__restrict__
   }
 }
 
+/* LDS of the gradient kernel: W1 (read per tile: in registers it would
+ * spill at two wavefronts per SIMD), W2 and W2^T shared by the workgroup, a
+ * transpose buffer per wavefront; after the tiles the W2/transpose bytes carry
+ * the partial sums of wavefronts 1.. to wavefront 0 (GRAD_XCHG floats per
+ * lane). */
+#define GRAD_XCHG 94
+struct __align__(16) MlpGradLds {
+  uint16_t w1[MLP_H * MLP_F];
+  union {
+    struct {
+      uint16_t w2[MLP_H * MLP_HS];
+      uint16_t w2t[MLP_H * MLP_HS];
+      uint16_t tr[GRAD_WAVES][16 * MLP_HS];
+    } m;
+    float xchg[(GRAD_WAVES - 1) * GRAD_XCHG * 64];
+  };
+};
+
 /* Gradient of sum_e BCE(sigmoid(z_e), y_e) * inv_e and, when want_loss, the
- * BCE sum (zero otherwise), one row of PROW partial sums per wavefront
- * (row = blockIdx.x).  The next tile's features and labels are loaded while
- * the current tile computes. */
-__global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
-                                                 const uint8_t *__restrict__ Y, long E,
-                                                 float inv_e, const float *__restrict__ w,
-                                                 const uint16_t *__restrict__ wb,
-                                                 float *__restrict__ part, int want_loss) {
-  __shared__ MlpLds L;
-  const int l = threadIdx.x, g = l >> 4, c = l & 15;
+ * BCE sum (zero otherwise), one row of PROW partial sums per workgroup
+ * (row = blockIdx.x).  GRAD_WAVES wavefronts per workgroup (two per SIMD fit
+ * the register budget) take tiles gw, gw + total, ... (gw = the global
+ * wavefront index); their sums meet in LDS, added in wavefront order, before
+ * wavefront 0 writes the row.  The next tile's features and labels are loaded
+ * while the current tile computes. */
+__global__ void __launch_bounds__(64 * GRAD_WAVES, 2)
+    k_mlp_grad(const uint16_t *__restrict__ X, const uint8_t *__restrict__ Y, long E,
+               float inv_e, const float *__restrict__ w, const uint16_t *__restrict__ wb,
+               float *__restrict__ part, int want_loss) {
+  __shared__ MlpGradLds L;
+  const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, g = l >> 4, c = l & 15;
+  uint16_t *tr = L.m.tr[wave];
   LaneW W;
   load_lane_w(W, w, wb, l);
-  stage_w2(L, wb, l);
+  stage_w2(L.m.w2, L.m.w2t, wb, threadIdx.x);
+  for (int i = threadIdx.x; i < MLP_H * MLP_F; i += blockDim.x) L.w1[i] = wb[BW1 + i];
+  __syncthreads();
   fl4 acc1[4], acc2[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -327,14 +361,20 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
   }
   float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gw3[4] = {0, 0, 0, 0};
   float gb3 = 0.0f, loss = 0.0f;
-  long tile = blockIdx.x;
+  const long stride = (long)gridDim.x * GRAD_WAVES;
+  long tile = (long)blockIdx.x * GRAD_WAVES + wave;
   TileIn in = load_tile(X, Y, E, tile * 16, l);
-  for (; tile * 16 < E; tile += gridDim.x) {
+  for (; tile * 16 < E; tile += stride) {
     const long e0 = tile * 16;
-    const TileIn next = load_tile(X, Y, E, (tile + gridDim.x) * 16, l);
+    const TileIn next = load_tile(X, Y, E, (tile + stride) * 16, l);
     fl4 z1[4], z2[4], z3;
     bf4 h1b[4];
-    tile_forward(W, tile_xa(in, E, e0, l), L, l, z1, h1b, z2, z3);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      const bf4 v = *reinterpret_cast<const bf4 *>(&L.w1[(16 * nt + c) * MLP_F + 4 * (g & 1)]);
+      W.w1[nt] = g < 2 ? v : bf4{0, 0, 0, 0};
+    }
+    tile_forward(W, tile_xa(in, E, e0, l), L.m.w2, tr, l, z1, h1b, z2, z3);
     fl4 dz3;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -372,13 +412,13 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) L.tr[(4 * g + r) * MLP_H + 16 * nt + c] = (uint16_t)dz2b[nt][r];
+      for (int r = 0; r < 4; ++r) tr[(4 * g + r) * MLP_HS + 16 * nt + c] = (uint16_t)dz2b[nt][r];
     wave_sync();
     const bf4 xb = tile_xb(in, E, e0, l);
     bf4 da[4];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
-      da[ks] = *reinterpret_cast<const bf4 *>(&L.tr[c * MLP_H + 16 * ks + 4 * g]);
+      da[ks] = *reinterpret_cast<const bf4 *>(&tr[c * MLP_HS + 16 * ks + 4 * g]);
     wave_sync();
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
@@ -386,7 +426,7 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks)
         dh = mfma16(da[ks],
-                    *reinterpret_cast<const bf4 *>(&L.w2t[(16 * nt + c) * MLP_H + 16 * ks + 4 * g]),
+                    *reinterpret_cast<const bf4 *>(&L.m.w2t[(16 * nt + c) * MLP_HS + 16 * ks + 4 * g]),
                     dh);
       fl4 dz1;
       float sb = 0.0f;
@@ -400,6 +440,56 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t *__restrict__ X,
       acc1[nt] = mfma16(pack4(dz1[0], dz1[1], dz1[2], dz1[3]), xb, acc1[nt]);
     }
     in = next;
+  }
+  /* wavefronts 1.. hand their sums to wavefront 0 through LDS (the weight
+   * and transpose buffers are dead from here) */
+  __syncthreads();
+  if (wave > 0) {
+    float *x = L.xchg + (size_t)(wave - 1) * GRAD_XCHG * 64 + l;
+    int v = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[64 * v++] = acc2[a][b][r];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x[64 * v++] = acc1[a][r];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      x[64 * v++] = gb1[a];
+      x[64 * v++] = gb2[a];
+      x[64 * v++] = gw3[a];
+    }
+    x[64 * v++] = gb3;
+    x[64 * v++] = loss;
+  }
+  __syncthreads();
+  if (wave > 0) return;
+#pragma unroll
+  for (int q = 0; q < GRAD_WAVES - 1; ++q) {
+    const float *x = L.xchg + (size_t)q * GRAD_XCHG * 64 + l;
+    int v = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc2[a][b][r] += x[64 * v++];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc1[a][r] += x[64 * v++];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      gb1[a] += x[64 * v++];
+      gb2[a] += x[64 * v++];
+      gw3[a] += x[64 * v++];
+    }
+    gb3 += x[64 * v++];
+    loss += x[64 * v++];
   }
   float *row = part + (size_t)blockIdx.x * PROW;
 #pragma unroll
@@ -655,8 +745,8 @@ static int prior_launch_forward(AcoHipPrior *pr, hipStream_t st, const uint16_t 
 static int prior_launch_grad(AcoHipPrior *pr, const uint16_t *X, const uint8_t *Y, int64_t E,
                              float lr_t, int want_loss) {
   const long tiles = (long)((E + 15) / 16);
-  const int rows = (int)std::max<long>(1, std::min<long>(tiles, MLP_GRID));
-  hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(64), 0, pr->stream, X, Y, (long)E,
+  const int rows = (int)std::max<long>(1, std::min<long>((tiles + GRAD_WAVES - 1) / GRAD_WAVES, MLP_GRID));
+  hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(64 * GRAD_WAVES), 0, pr->stream, X, Y, (long)E,
                      1.0f / (float)E, pr->d_w, pr->d_wb, pr->d_part, want_loss);
   PLAUNCHED(pr, "k_mlp_grad");
   hipLaunchKernelGGL(k_mlp_reduce_rows, dim3((PROW + 255) / 256, RED_SPLIT), dim3(256), 0,
