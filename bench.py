@@ -216,7 +216,7 @@ def large_clustered_leg(epochs: int = 20):
                               slack_pct=105, seed=1)
     with HipSolver(inst, Params(cand_k=32, ants=1024, rho=0.1), seeds=[1]) as s:
         s.prepare()
-        shapes = tuner.tune_shapes(s, cache=None, launches=2)
+        shapes = tuner.tune_shapes(s, cache=None, launches=4)
         s.run(fixed_epochs=3, chunk_epochs=3)
         s.sync()
         t0 = time.perf_counter()
