@@ -104,6 +104,40 @@ def reference_openmp_baseline(seconds_budget=8.0, width20=False):
                       + f"), {t:.2f} s solver time"}
 
 
+def reference_sequential_baseline(seconds_budget=8.0):
+    """The reference sequential CLI (aco_sequential.c, built from its sources
+    by oracle/ref.mk; candidate list = all n customers, one core) on the same
+    unit-demand n=1000, K=8 instance and 256 ants, stopped by its timeout:
+    epochs completed over the elapsed time of its last progress line.  None
+    when the binary was not built."""
+    import re
+    import subprocess
+    import tempfile
+    from vehicleroutingproblem_amd import Instance
+    exe = os.path.join(ROOT, "oracle", "_ref", "aco_seq")
+    if not os.path.exists(exe):
+        return None
+    inst = Instance.uniform(N_CUST, K_VEH, side=1000, dmin=1, dmax=1, slack_pct=120, seed=1)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "unit_n1000_k8.vrp")
+        inst.write(path)
+        env = dict(os.environ, ACO_SOLVER_TIMEOUT_SECONDS=str(seconds_budget),
+                   ACO_SOLVER_PROGRESS_INTERVAL_SECONDS="0.5")
+        out = subprocess.run([exe, path, str(K_VEH), str(ANTS), "1"], env=env,
+                             capture_output=True, text=True, timeout=300).stdout
+    prog = re.findall(r"epoca=([0-9]+) .*?tempo_trascorso=([0-9.]+)s", out)
+    if not prog:
+        return None
+    epochs, t = int(prog[-1][0]), float(prog[-1][1])
+    if t <= 0:
+        return None
+    return {"value": round(ANTS * epochs / t, 1), "unit": "ant-tours/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"reference src/seq CLI, {epochs} epochs x {ANTS} ants in {t:.1f} s "
+                      "(its timeout stop), unit-demand n=1000 K=8, candidate list = all "
+                      "customers"}
+
+
 # algorithmic flops per edge and training step: forward 2(8*64 + 64*64 + 64),
 # backward 2(64*64 [dW2] + 64*64 [dH1] + 64*8 [dW1] + 64 [dW3])
 PRIOR_FLOPS_PER_EDGE_STEP = 2 * (8 * 64 + 64 * 64 + 64) + 2 * (64 * 64 * 2 + 64 * 8 + 64)
@@ -498,6 +532,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * dt / epochs, 4),
+        "epochs_per_s": round(epochs / dt, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -549,6 +584,9 @@ def main():
         ref20 = reference_openmp_baseline(width20=True)
         if ref20:
             out["cpu_baseline"]["reference_openmp_width20"] = ref20
+        seq = reference_sequential_baseline()
+        if seq:
+            out["cpu_baseline"]["reference_sequential"] = seq
         if not args.no_quality:
             out["cpu_baseline"]["quality_vs_reference_seq"] = quality_leg(32)
     print(json.dumps(out))
