@@ -30,6 +30,9 @@ struct LaunchBucket {
   int W = 4, stage = 1, eta_lds = 1, rows = 1, groups = 1;
   size_t lds = 0;
   float tuned_ms = 0.0f;     /* construct time measured by the autotuner, 0 = not tuned */
+  size_t off_at = 0;         /* its item offsets at d_boff[off_at], slot offsets after them */
+  long long flat_total = 0;  /* its evaporation items + slots */
+  int sd_blocks = 1;         /* select/deposit workgroups per colony */
 };
 
 struct AcoHipContext {
@@ -79,6 +82,7 @@ struct AcoHipContext {
   DevCtx P{};
   /* timing */
   bool timing = false;
+  bool serial_updates = false; /* ACO_HIP_SERIAL_UPDATES=1: context-wide update kernels */
   std::vector<TimingRec> pending;
   std::vector<hipEvent_t> free_events;
   AcoHipTiming t{};
@@ -90,6 +94,7 @@ struct AcoHipContext {
   int *d_colmap = nullptr;
   int *d_feas = nullptr;                   /* per colony: 1 + last epoch with a feasible ant */
   long long *d_item_off = nullptr, *d_slot_off = nullptr; /* flat evaporation prefix offsets */
+  long long *d_boff = nullptr; /* the same per size bucket: item then slot offsets */
   long long flat_total = 0;
   std::vector<hipStream_t> bstreams; /* streams of buckets 1.. */
   std::vector<hipEvent_t> bevents;   /* fork event, then one join event per extra bucket */

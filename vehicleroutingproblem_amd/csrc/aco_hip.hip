@@ -73,8 +73,9 @@ static void free_pools(AcoHipContext *c) {
   if (c->d_feas) (void)hipFree(c->d_feas);
   if (c->d_item_off) (void)hipFree(c->d_item_off);
   if (c->d_slot_off) (void)hipFree(c->d_slot_off);
+  if (c->d_boff) (void)hipFree(c->d_boff);
   c->d_feas = nullptr;
-  c->d_item_off = c->d_slot_off = nullptr;
+  c->d_item_off = c->d_slot_off = c->d_boff = nullptr;
   for (hipStream_t st : c->bstreams) (void)hipStreamDestroy(st);
   for (hipEvent_t ev : c->bevents) (void)hipEventDestroy(ev);
   c->bstreams.clear();
@@ -445,6 +446,9 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if (c->sd_eta) sd += r16((size_t)c->eta_len * 4);
   c->sd_lds = (sd <= 131072 && c->max_stride <= 65536) ? sd : 0;
   if (env_int("ACO_HIP_FUSED_UPDATE", 0)) c->sd_lds = 0; /* force the fused update kernel */
+  /* force the context-wide update kernels after all buckets (instead of one
+   * construct/evaporate/deposit chain per bucket) */
+  c->serial_updates = env_int("ACO_HIP_SERIAL_UPDATES", 0) != 0;
   /* workgroups per colony: enough for 256 pair events each, but about 2048 in
    * all (a batch of many colonies fills the chip with one or a few each, and
    * every workgroup stages both tours) */
@@ -483,6 +487,28 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
                       hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->d_slot_off, so.data(), sizeof(long long) * so.size(),
                       hipMemcpyHostToDevice));
+    /* the same offsets per size bucket, over the bucket's colony list */
+    std::vector<long long> bo;
+    for (LaunchBucket &lb : c->buckets) {
+      const int nc = (int)lb.cols.size();
+      lb.off_at = bo.size();
+      long long ia = 0, sa = 0;
+      std::vector<long long> bi(nc + 1, 0), bs(nc + 1, 0);
+      for (int q = 0; q < nc; ++q) {
+        const int col = lb.cols[q];
+        ia += io[col + 1] - io[col];
+        sa += so[col + 1] - so[col];
+        bi[q + 1] = ia;
+        bs[q + 1] = sa;
+      }
+      lb.flat_total = ia + sa;
+      lb.sd_blocks = std::max(1, std::min({64, (2 * c->max_stride + 255) / 256,
+                                           (2048 + nc - 1) / std::max(nc, 1)}));
+      bo.insert(bo.end(), bi.begin(), bi.end());
+      bo.insert(bo.end(), bs.begin(), bs.end());
+    }
+    if ((rc = dalloc(c, &c->d_boff, sizeof(long long) * bo.size()))) return rc;
+    HIPC(c, hipMemcpy(c->d_boff, bo.data(), sizeof(long long) * bo.size(), hipMemcpyHostToDevice));
   }
   {
     std::vector<int> cm;
@@ -846,18 +872,27 @@ static int launch_evap_update(AcoHipContext *c) {
   return ACO_HIP_OK;
 }
 
+/* Flat evaporation of `ncol` colonies (offsets item_off/slot_off; colony
+ * list cmap, or all colonies in order when null) on stream st. */
+static void enqueue_evap_flat(AcoHipContext *c, hipStream_t st, const long long *item_off,
+                              const long long *slot_off, int ncol, const int *cmap,
+                              long long total) {
+  const long long blocks = std::max(1LL, std::min((total + 255) / 256, 16384LL));
+  /* the colony offsets (up to EVAP_LDS_COLONIES colonies), the taupow
+   * table, and the eta table while it is at most 32 KB, in LDS */
+  const int eta_lds = c->eta_len <= 8192 ? 1 : 0;
+  const size_t off_lds =
+      ncol <= EVAP_LDS_COLONIES ? 2 * sizeof(long long) * ((size_t)ncol + 1) : 0;
+  const size_t lds = off_lds + 256 * 4 + (eta_lds ? (size_t)c->eta_len * 4 : 0);
+  hipLaunchKernelGGL(k_evaporate_flat, dim3((unsigned)blocks), dim3(256), lds, st, c->P, item_off,
+                     slot_off, ncol, eta_lds, cmap);
+}
+
 static int launch_evap_flat(AcoHipContext *c) {
-  const long long blocks = std::max(1LL, std::min((c->flat_total + 255) / 256, 16384LL));
   {
     Timed tm(c, T_EVAP);
-    /* the colony offsets (up to EVAP_LDS_COLONIES colonies), the taupow
-     * table, and the eta table while it is at most 32 KB, in LDS */
-    const int eta_lds = c->eta_len <= 8192 ? 1 : 0;
-    const size_t off_lds =
-        c->num_col <= EVAP_LDS_COLONIES ? 2 * sizeof(long long) * ((size_t)c->num_col + 1) : 0;
-    const size_t lds = off_lds + 256 * 4 + (eta_lds ? (size_t)c->eta_len * 4 : 0);
-    hipLaunchKernelGGL(k_evaporate_flat, dim3((unsigned)blocks), dim3(256), lds, c->stream, c->P,
-                       c->d_item_off, c->d_slot_off, c->num_col, eta_lds);
+    enqueue_evap_flat(c, c->stream, c->d_item_off, c->d_slot_off, c->num_col, nullptr,
+                      c->flat_total);
   }
   LAUNCHED(c, "k_evaporate_flat");
   return ACO_HIP_OK;
@@ -867,9 +902,39 @@ static int launch_select_deposit(AcoHipContext *c) {
   {
     Timed tm(c, T_BEST);
     hipLaunchKernelGGL(k_select_deposit, dim3(c->sd_blocks, c->num_col), dim3(256), c->sd_lds,
-                       c->stream, c->P, c->sd_eta);
+                       c->stream, c->P, c->sd_eta, nullptr);
   }
   LAUNCHED(c, "k_select_deposit");
+  return ACO_HIP_OK;
+}
+
+/* One epoch of a multi-bucket context as independent chains: every size
+ * bucket constructs, evaporates and deposits its own colonies on its own
+ * stream (bucket 0 on the context stream), forked from and joined back to
+ * the context stream.  A bucket's update kernels then run while larger
+ * buckets are still constructing.  Colonies never share state, so the result
+ * is the same as with the context-wide update kernels. */
+static int launch_epoch_pipelined(AcoHipContext *c) {
+  const bool stats = c->P.stats != nullptr;
+  const size_t nb = c->buckets.size();
+  HIPC(c, hipEventRecord(c->bevents[0], c->stream));
+  for (size_t b = 1; b < nb; ++b) HIPC(c, hipStreamWaitEvent(c->bstreams[b - 1], c->bevents[0], 0));
+  for (size_t b = 0; b < nb; ++b) {
+    const LaunchBucket &lb = c->buckets[b];
+    hipStream_t st = b == 0 ? c->stream : c->bstreams[b - 1];
+    const int nc = (int)lb.cols.size();
+    const int *cmap = c->d_colmap + lb.col_off;
+    launch_bucket(c, lb, st, stats);
+    enqueue_evap_flat(c, st, c->d_boff + lb.off_at, c->d_boff + lb.off_at + nc + 1, nc, cmap,
+                      lb.flat_total);
+    hipLaunchKernelGGL(k_select_deposit, dim3(lb.sd_blocks, nc), dim3(256), c->sd_lds, st, c->P,
+                       c->sd_eta, cmap);
+  }
+  for (size_t b = 1; b < nb; ++b) {
+    HIPC(c, hipEventRecord(c->bevents[b], c->bstreams[b - 1]));
+    HIPC(c, hipStreamWaitEvent(c->stream, c->bevents[b], 0));
+  }
+  LAUNCHED(c, "k_construct/k_evaporate_flat/k_select_deposit");
   return ACO_HIP_OK;
 }
 
@@ -906,9 +971,16 @@ static int launch_epochs(AcoHipContext *c, int count) {
   int rc;
   for (int e = 0; e < count; ++e) {
     /* construct, then evaporation (it reads feasibility from the ants),
-     * then best selection + both deposits over many workgroups; instances
-     * whose tours do not fit the LDS tables of that kernel use the fused
-     * evaporate + single-workgroup update kernel instead */
+     * then best selection + both deposits over many workgroups - one such
+     * chain per size bucket when there are several (unless timing, which
+     * measures each phase over all buckets, or ACO_HIP_SERIAL_UPDATES asks
+     * for the context-wide order); instances whose tours do not fit the LDS
+     * tables of that kernel use the fused evaporate + single-workgroup
+     * update kernel instead */
+    if (c->sd_lds && c->buckets.size() > 1 && !c->timing && !c->serial_updates) {
+      if ((rc = launch_epoch_pipelined(c))) return rc;
+      continue;
+    }
     if ((rc = launch_construct(c))) return rc;
     if (c->sd_lds) {
       if ((rc = launch_evap_flat(c))) return rc;

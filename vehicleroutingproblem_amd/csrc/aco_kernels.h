@@ -146,11 +146,11 @@ template <bool STAGE, bool STATS, int ROWS>
 __global__ void k_construct(DevCtx P, int groups_per_colony, int W, const int *cols);
 __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
 __global__ void k_evaporate_flat(DevCtx P, const long long *item_off, const long long *slot_off,
-                                 int ncol, int eta_lds);
+                                 int ncol, int eta_lds, const int *cmap);
 __global__ void k_refresh_scores(DevCtx P);
 __global__ void k_update(DevCtx P, int mode);
 __global__ void k_evap_update(DevCtx P, int mat_blocks);
-__global__ void k_select_deposit(DevCtx P, int stage_eta);
+__global__ void k_select_deposit(DevCtx P, int stage_eta, const int *cmap);
 
 /* LDS bytes of k_select_deposit: both tours (int32) and the position of
  * every customer in each (u16). */

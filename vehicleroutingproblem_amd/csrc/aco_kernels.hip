@@ -747,7 +747,8 @@ __device__ __forceinline__ int colony_of(const long long *off, int n, long long 
 /* Evaporation of every colony of the context in one flat grid: items
  * [0, item_off[ncol]) are the colonies' matrix items (all-pairs weight items
  * of fs_item, or 16-byte pheromone vectors without the weight matrix),
- * items after them the candidate slots.  Each workgroup takes a contiguous
+ * items after them the candidate slots.  With cmap, offset entry i is colony
+ * cmap[i] (the colonies of one size bucket).  Each workgroup takes a contiguous
  * range of items and its threads stride through it by the workgroup size, so
  * a thread's consecutive items nearly always belong to one colony: the
  * colony (a binary search of the prefix offsets, staged in LDS when there are
@@ -757,7 +758,7 @@ __device__ __forceinline__ int colony_of(const long long *off, int n, long long 
  * epoch by k_construct) - the condition of k_evaporate with from_ants. */
 __global__ void __launch_bounds__(256) k_evaporate_flat(DevCtx P, const long long *item_off,
                                                         const long long *slot_off, int ncol,
-                                                        int eta_lds) {
+                                                        int eta_lds, const int *cmap) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dyn_smem[];
   long long *s_ioff = reinterpret_cast<long long *>(dyn_smem);
   long long *s_soff = s_ioff + (ncol + 1);
@@ -791,9 +792,10 @@ __global__ void __launch_bounds__(256) k_evaporate_flat(DevCtx P, const long lon
       mat = g < ti;
       const long long *off = mat ? ioff : soff;
       const long long base = mat ? 0 : ti;
-      const int c = colony_of(off, ncol, g - base);
-      lo = base + off[c];
-      hi = base + off[c + 1];
+      const int ci = colony_of(off, ncol, g - base);
+      lo = base + off[ci];
+      hi = base + off[ci + 1];
+      const int c = cmap ? cmap[ci] : ci;
       const AcoHipColonyState *S = P.state + c;
       ok = !S->halted && P.feas[c] == S->epoch + 1;
       D = P.col[c];
@@ -1112,12 +1114,14 @@ __device__ __forceinline__ void pair_apply(const UpdTables &T, bool fwd_x, float
  * no entries.  The empty-route (0,0) entry is one thread's.  The last
  * workgroup to finish (device counter) commits the colony state, the
  * global-best tour and the stop rules (aco_cuda.cu:104-126,331-399). */
-__global__ void __launch_bounds__(256) k_select_deposit(DevCtx P, int stage_eta) {
+__global__ void __launch_bounds__(256) k_select_deposit(DevCtx P, int stage_eta,
+                                                        const int *cmap) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dyn_smem[];
   __shared__ unsigned long long s_min[4];
   __shared__ int s_cnt[4], s_last, s_zz[2];
   __shared__ float s_deq[256], s_thr[256], s_tp[256];
-  const int colony = blockIdx.y;
+  /* grid row y: colony y, or colony cmap[y] of a size bucket's list */
+  const int colony = cmap ? cmap[blockIdx.y] : (int)blockIdx.y;
   AcoHipColonyState *S = P.state + colony;
   const ColonyDesc D = P.col[colony];
   const InstDesc I = P.inst[D.inst];
