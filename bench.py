@@ -144,15 +144,15 @@ PRIOR_FLOPS_PER_EDGE_STEP = 2 * (8 * 64 + 64 * 64 + 64) + 2 * (64 * 64 * 2 + 64 
 
 
 def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
-    """Learned heuristic prior against plain ACO at equal wall-clock on
+    """Batched learned ACO against batched plain ACO at equal wall-clock on
     n=2000, K=16, 256 ants, k=16 instances (half uniform, half clustered):
     the prior trains on elite tours of `train_count` seeded n in [200, 2000]
-    instances (harvested with plain ACO, 100 epochs, 64 ants), then each test
-    instance runs `budget_s` seconds with and without it (the learned run's
-    budget includes the prior's feature pass and inference)."""
-    import math
+    instances (harvested with plain ACO, 100 epochs, 64 ants), then the
+    `seeds` test instances run together in one batched context for `budget_s`
+    seconds, once plain and once with the prior applied to every instance (the
+    learned run's budget includes the prior's feature pass and inference)."""
     from vehicleroutingproblem_amd import HipSolver, Instance, Params
-    from vehicleroutingproblem_amd.prior import EdgePrior, train_prior, training_instances
+    from vehicleroutingproblem_amd.prior import train_prior, training_instances
     t0 = time.perf_counter()
     train = training_instances(train_count, seed=7)
     prior = train_prior(train, epochs=100, steps=0)
@@ -163,29 +163,37 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
     prior.train(steps, lr=2e-3)
     t_train = time.perf_counter() - t0
     flops = edges * steps * PRIOR_FLOPS_PER_EDGE_STEP
-    plain, learned = [], []
+    test = []
     for t in range(seeds):
         if t % 2 == 0:
-            inst = Instance.uniform(2000, 16, side=1000, dmin=1, dmax=10, slack_pct=110,
-                                    seed=500 + t)
+            test.append(Instance.uniform(2000, 16, side=1000, dmin=1, dmax=10, slack_pct=110,
+                                         seed=500 + t))
         else:
-            inst = Instance.clustered(2000, 16, side=1000, clusters=25, sigma=30, dmin=1,
-                                      dmax=10, slack_pct=110, seed=500 + t)
-        for use in (False, True):
-            with HipSolver(inst, Params(cand_k=16, ants=256, rho=0.1), seeds=[77 + t]) as sv:
-                sv.prepare()
-                ts = time.perf_counter()
-                if use:
-                    prior.apply(sv)
-                sv.run(timeout_seconds=budget_s - (time.perf_counter() - ts), max_epochs=10 ** 6,
-                       chunk_epochs=16)
-                (learned if use else plain).append(sv.best()[1])
+            test.append(Instance.clustered(2000, 16, side=1000, clusters=25, sigma=30, dmin=1,
+                                           dmax=10, slack_pct=110, seed=500 + t))
+    res = {}
+    for use in (False, True):
+        with HipSolver(test, Params(cand_k=16, ants=256, rho=0.1),
+                       seeds=[77 + t for t in range(seeds)]) as sv:
+            sv.prepare()
+            sv.sync()
+            ts = time.perf_counter()
+            if use:
+                for i in range(seeds):
+                    prior.apply(sv, instance=i)
+            sv.run(timeout_seconds=budget_s - (time.perf_counter() - ts), max_epochs=10 ** 6,
+                   chunk_epochs=16)
+            res[use] = ([sv.best(i)[1] for i in range(seeds)], sv.colony_state(0)["epoch"])
     prior.close()
+    plain, learned = res[False][0], res[True][0]
     gains = [(a - b) / a for a, b in zip(plain, learned)]
-    return {"test": "n=2000 K=16 m=256 k=16, half uniform / half clustered (sigma 30)",
+    return {"test": f"{seeds} instances n=2000 K=16 m=256 k=16 in one batched context, half "
+                    "uniform / half clustered (sigma 30)",
             "budget_s": budget_s, "seeds": seeds,
             "plain_best": plain, "learned_best": learned,
+            "plain_epochs": res[False][1], "learned_epochs": res[True][1],
             "mean_gain_pct": round(100 * sum(gains) / len(gains), 3),
+            "instances_improved": sum(1 for g in gains if g > 0),
             "train_instances": train_count, "train_edges": edges,
             "harvest_s": round(t_harvest, 3), "train_steps": steps,
             "train_s": round(t_train, 3),
@@ -571,7 +579,7 @@ def main():
         out["batched"] = batched_leg(20)
         out["large_clustered"] = large_clustered_leg(20)
     if not args.no_prior:
-        out["learned_prior"] = learned_prior_leg(args.prior_seconds, 4, 64)
+        out["learned_prior"] = learned_prior_leg(args.prior_seconds, 8, 64)
     if not args.no_campaign:
         out["reference_campaign"] = reference_campaign_leg(2.0)
     if not args.no_cpu:

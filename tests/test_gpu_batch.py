@@ -111,3 +111,26 @@ def test_many_colonies_match_oracle(gpu):
         assert np.array_equal(cg, co), (i, c)
         assert b.colony_state(2 * i + c)["global_best_cost"] == o.state()["global_best_cost"], (i, c)
     b.close()
+
+
+def test_bucket_chains_match_context_wide_updates(gpu, monkeypatch):
+    """One construct/evaporate/deposit chain per size bucket (the default for
+    several buckets) gives the same tours, pheromone and states as the
+    context-wide update kernels after all buckets (ACO_HIP_SERIAL_UPDATES=1)."""
+    insts, ks, seeds = _batch()
+    p = Params(cand_k=16, ants=32, colonies=2, rho=0.1)
+    monkeypatch.delenv("ACO_HIP_SERIAL_UPDATES", raising=False)
+    a = HipSolver(insts, p, seeds=seeds, cand_k=ks)
+    monkeypatch.setenv("ACO_HIP_SERIAL_UPDATES", "1")
+    b = HipSolver(insts, p, seeds=seeds, cand_k=ks)
+    assert len(a.buckets()) == 4
+    for s in (a, b):
+        s.prepare()
+        s.run(fixed_epochs=5, chunk_epochs=5)
+    for c in range(a.num_colonies):
+        for x, y in zip(a.tours(c), b.tours(c)):
+            assert np.array_equal(x, y)
+        assert np.array_equal(a.tau(c)[0], b.tau(c)[0]) and np.array_equal(a.tau(c)[1], b.tau(c)[1])
+        assert a.colony_state(c) == b.colony_state(c)
+    a.close()
+    b.close()
