@@ -155,7 +155,7 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
     from vehicleroutingproblem_amd.prior import train_prior, training_instances
     t0 = time.perf_counter()
     train = training_instances(train_count, seed=7)
-    prior = train_prior(train, epochs=100, steps=0)
+    prior = train_prior(train, epochs=100, steps=0, batch=512)
     t_harvest = time.perf_counter() - t0
     edges = prior.size()
     steps = 600
@@ -442,6 +442,9 @@ def main():
                     help="skip the learned-prior vs plain ACO leg")
     ap.add_argument("--prior-seconds", type=float, default=3.0,
                     help="wall-clock budget per test run of the learned-prior leg")
+    ap.add_argument("--prior-train-count", type=int, default=4096,
+                    help="seeded n in [200, 2000] instances whose elite tours train the "
+                         "learned prior")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device twice (calibrates the HBM "
                          "byte counters of a profiler against a copy of known size)")
@@ -579,7 +582,7 @@ def main():
         out["batched"] = batched_leg(20)
         out["large_clustered"] = large_clustered_leg(20)
     if not args.no_prior:
-        out["learned_prior"] = learned_prior_leg(args.prior_seconds, 8, 64)
+        out["learned_prior"] = learned_prior_leg(args.prior_seconds, 8, args.prior_train_count)
     if not args.no_campaign:
         out["reference_campaign"] = reference_campaign_leg(2.0)
     if not args.no_cpu:
