@@ -267,12 +267,41 @@ def large_clustered_leg(epochs: int = 20):
         dt = time.perf_counter() - t0
         done = s.colony_state()["epoch"] - 3
         best = s.colony_state()["global_best_cost"]
+        # per-kernel durations: 8 more epochs, eager launches with hipEvents
+        s.set_timing(True)
+        s.run(fixed_epochs=3 + epochs + 8, chunk_epochs=8, use_graph=False)
+        s.sync()
+        t = s.timing()
+        s.set_timing(False)
+        try:  # the all-pairs fallback weights exist iff the matrix is on
+            s.fallback_weights(0)
+            fs_on = True
+        except Exception:
+            fs_on = False
+    ep = max(1, t["construct_launches"])
+    evap_ms = t["evaporate_ms"] / max(1, t["evaporate_launches"])
+    # evaporation of the (n+1) x S pheromone matrix (S = n+1 rounded up to 16)
+    # in 16-byte items: each reads and writes its 16 levels and, with the
+    # all-pairs fallback weights on, writes their 16 fp32 weights
+    n1 = inst.n + 1
+    items = n1 * ((n1 + 15) // 16)
+    evap_bytes = items * (32 + (64 if fs_on else 0))
+    evap_gbs = evap_bytes / (evap_ms * 1e-3) / 1e9
     return {"workload": "n=5000 K=40 clustered (sigma 300 in [0,10000]^2), demands U[1,100], "
                         "Q=ceil(1.05 sum/40), 1024 ants, k=32",
             "value": round(1024 * done / dt, 1), "unit": "ant-tours/s",
             "ms_per_epoch": round(1e3 * dt / max(done, 1), 3), "epochs": done,
             "best_cost_after": best,
-            "ants_per_group": shapes[0]["ants_per_group"], "stage_lds": shapes[0]["stage_lds"]}
+            "ants_per_group": shapes[0]["ants_per_group"], "stage_lds": shapes[0]["stage_lds"],
+            "kernel_ms_per_epoch": {
+                "construct": round(t["construct_ms"] / ep, 4),
+                "evaporate": round(t["evaporate_ms"] / ep, 4),
+                "select_and_deposit": round((t["best_ms"] + t["deposit_ms"]) / ep, 4)},
+            "evaporation_roofline": {
+                "bound": "hbm", "kernel": "k_evaporate_flat", "achieved": round(evap_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(evap_gbs / HBM_PEAK_GBS, 4),
+                "bytes_per_launch": evap_bytes, "avg_launch_ms": round(evap_ms, 4),
+                "pheromone_entries": n1 * n1, "fallback_matrix": bool(fs_on)}}
 
 
 def real_tour_cost(inst, tour) -> float:

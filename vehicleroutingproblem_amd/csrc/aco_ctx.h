@@ -37,6 +37,7 @@ struct LaunchBucket {
 
 struct AcoHipContext {
   int device = 0;
+  int num_cu = 256; /* compute units of the device */
   hipStream_t stream = nullptr;
   char err[512] = {0};
   bool loaded = false;

@@ -178,6 +178,10 @@ extern "C" int aco_hip_create(AcoHipContext **out, int device) {
     delete c;
     return ACO_HIP_ERR_BACKEND;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      cus > 0)
+    c->num_cu = cus;
   *out = c;
   return ACO_HIP_OK;
 }
@@ -618,6 +622,9 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     HIPC(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
   HIPC(c, hipFuncSetAttribute((const void *)k_select_deposit,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+  /* colony offsets (<= 16 KB) + taupow + an eta table of up to 64 KB */
+  HIPC(c, hipFuncSetAttribute((const void *)k_evaporate_flat,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 98304));
   c->loaded = true;
   return ACO_HIP_OK;
 }
@@ -877,10 +884,14 @@ static int launch_evap_update(AcoHipContext *c) {
 static void enqueue_evap_flat(AcoHipContext *c, hipStream_t st, const long long *item_off,
                               const long long *slot_off, int ncol, const int *cmap,
                               long long total) {
-  const long long blocks = std::max(1LL, std::min((total + 255) / 256, 16384LL));
   /* the colony offsets (up to EVAP_LDS_COLONIES colonies), the taupow
-   * table, and the eta table while it is at most 32 KB, in LDS */
-  const int eta_lds = c->eta_len <= 8192 ? 1 : 0;
+   * table, and the eta table while it is at most 64 KB, in LDS.  Every
+   * workgroup stages the eta table once: past 8 KB the grid is two
+   * workgroups per CU (what fits the LDS), each walking a long item range,
+   * so the staging stays small beside the matrix traffic. */
+  const int eta_lds = c->eta_len <= 16384 ? 1 : 0;
+  const long long cap = (eta_lds && c->eta_len > 2048) ? 2LL * c->num_cu : 16384LL;
+  const long long blocks = std::max(1LL, std::min((total + 255) / 256, cap));
   const size_t off_lds =
       ncol <= EVAP_LDS_COLONIES ? 2 * sizeof(long long) * ((size_t)ncol + 1) : 0;
   const size_t lds = off_lds + 256 * 4 + (eta_lds ? (size_t)c->eta_len * 4 : 0);

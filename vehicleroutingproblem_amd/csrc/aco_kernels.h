@@ -18,9 +18,11 @@
  *                  taupow[tau[u][v]] * eta[d(u,v)] for every pair, the weights
  *                  of the all-customer fallback roulette, kept in step with tau
  *                  by the same kernels (optional: NULL when too large)
- *   pher    uint8  per colony: (n+1)^2 pheromone matrix (row stride n+1,
- *                  the reference format), then the (n+1)*k candidate-aligned
- *                  copy tau_cand[i][t] == tau[i][cand[i][t]], 16-byte padded
+ *   pher    uint8  per colony: (n+1) x S pheromone matrix (row stride S, the
+ *                  rows of the reference's (n+1)^2 matrix padded to 16 bytes;
+ *                  the get/set entry points convert), then the (n+1)*k
+ *                  candidate-aligned copy tau_cand[i][t] == tau[i][cand[i][t]],
+ *                  16-byte padded
  *   tours   int32  [colonies][ants][stride] step-interleaved routes
  *   len/unvis int32, cost int64 [colonies*ants]
  *   best    int32  [colonies][stride]   global-best tour per colony
