@@ -887,15 +887,18 @@ static void enqueue_evap_flat(AcoHipContext *c, hipStream_t st, const long long 
   /* the colony offsets (up to EVAP_LDS_COLONIES colonies), the taupow
    * table, and the eta table while it is at most 64 KB, in LDS.  Every
    * workgroup stages the eta table once: past 8 KB the grid is two
-   * workgroups per CU (what fits the LDS), each walking a long item range,
-   * so the staging stays small beside the matrix traffic. */
+   * workgroups of 512 threads per CU (what fits the LDS; 16 wavefronts to
+   * cover the memory latency), each walking a long item range, so the
+   * staging stays small beside the matrix traffic. */
   const int eta_lds = c->eta_len <= 16384 ? 1 : 0;
-  const long long cap = (eta_lds && c->eta_len > 2048) ? 2LL * c->num_cu : 16384LL;
-  const long long blocks = std::max(1LL, std::min((total + 255) / 256, cap));
+  const bool big_eta = eta_lds && c->eta_len > 2048;
+  const long long cap = big_eta ? 2LL * c->num_cu : 16384LL;
+  const int threads = big_eta ? 512 : 256;
+  const long long blocks = std::max(1LL, std::min((total + threads - 1) / threads, cap));
   const size_t off_lds =
       ncol <= EVAP_LDS_COLONIES ? 2 * sizeof(long long) * ((size_t)ncol + 1) : 0;
   const size_t lds = off_lds + 256 * 4 + (eta_lds ? (size_t)c->eta_len * 4 : 0);
-  hipLaunchKernelGGL(k_evaporate_flat, dim3((unsigned)blocks), dim3(256), lds, st, c->P, item_off,
+  hipLaunchKernelGGL(k_evaporate_flat, dim3((unsigned)blocks), dim3(threads), lds, st, c->P, item_off,
                      slot_off, ncol, eta_lds, cmap);
 }
 

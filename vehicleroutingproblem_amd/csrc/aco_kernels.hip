@@ -756,7 +756,7 @@ __device__ __forceinline__ int colony_of(const long long *off, int n, long long 
  * when an item leaves the cached range.  A colony evaporates iff it is not
  * halted and some ant of this epoch was feasible (P.feas, tagged with the
  * epoch by k_construct) - the condition of k_evaporate with from_ants. */
-__global__ void __launch_bounds__(256) k_evaporate_flat(DevCtx P, const long long *item_off,
+__global__ void __launch_bounds__(512) k_evaporate_flat(DevCtx P, const long long *item_off,
                                                         const long long *slot_off, int ncol,
                                                         int eta_lds, const int *cmap) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dyn_smem[];
