@@ -248,14 +248,22 @@ def batched_leg(epochs: int, count: int = 512, ants: int = 128, fallback_matrix:
             "prepare_s": round(t_prep, 3), "autotune_s": round(t_tune, 3)}
 
 
+def large_clustered_instance():
+    """The n=5000 clustered workload: 25 Gaussian components, centres
+    U[0,10000]^2, sigma 300, depot at the centre, demands U[1,100], K=40,
+    Q=ceil(1.05 sum/40), seed 1."""
+    from vehicleroutingproblem_amd import Instance
+    return Instance.clustered(5000, 40, side=10000, clusters=25, sigma=300, dmin=1, dmax=100,
+                              slack_pct=105, seed=1)
+
+
 def large_clustered_leg(epochs: int = 20):
     """C3: n=5000 clustered (25 Gaussian components, centres U[0,10000]^2,
     sigma 300, depot at the centre), demands U[1,100], K=40,
     Q=ceil(1.05 sum/40), 1024 ants, k=32; `epochs` timed epochs after 3."""
-    from vehicleroutingproblem_amd import HipSolver, Instance, Params
+    from vehicleroutingproblem_amd import HipSolver, Params
     from vehicleroutingproblem_amd import tuner
-    inst = Instance.clustered(5000, 40, side=10000, clusters=25, sigma=300, dmin=1, dmax=100,
-                              slack_pct=105, seed=1)
+    inst = large_clustered_instance()
     with HipSolver(inst, Params(cand_k=32, ants=1024, rho=0.1), seeds=[1]) as s:
         s.prepare()
         shapes = tuner.tune_shapes(s, cache=None, launches=4)
@@ -480,7 +488,22 @@ def main():
     ap.add_argument("--train-probe", type=int, default=0, metavar="STEPS",
                     help="only harvest 16 training instances and train the edge prior "
                          "STEPS steps (the network kernels alone, for counter runs)")
+    ap.add_argument("--large-probe", type=int, default=0, metavar="EPOCHS",
+                    help="only run the n=5000 clustered workload (1024 ants, k=32, one ant "
+                         "per workgroup) for EPOCHS eager epochs (its kernels alone, for "
+                         "counter runs)")
     args = ap.parse_args()
+    if args.large_probe:
+        from vehicleroutingproblem_amd import HipSolver, Params
+        inst = large_clustered_instance()
+        with HipSolver(inst, Params(cand_k=32, ants=1024, rho=0.1, ants_per_group=1),
+                       seeds=[1]) as s:
+            s.prepare()
+            s.run(fixed_epochs=args.large_probe, chunk_epochs=args.large_probe, use_graph=False)
+            s.sync()
+            print(json.dumps({"large_probe_epochs": s.colony_state()["epoch"],
+                              "best_cost": s.colony_state()["global_best_cost"]}))
+        return
     if args.train_probe:
         from vehicleroutingproblem_amd.prior import train_prior, training_instances
         prior = train_prior(training_instances(16, seed=7), epochs=30, steps=0)
