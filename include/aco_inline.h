@@ -63,9 +63,28 @@ static inline ACO_HD float aco_sqrt_estimate(float x) {
 #endif
 }
 
+/* Products of operands below 2^23 whose result fits in 32 bits: one
+ * full-rate 24-bit multiply on the GPU (the 32-bit integer multiply runs at
+ * a quarter of the rate), a plain multiply elsewhere; exact either way. */
+static inline ACO_HD uint32_t aco_umul24(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul24(a, b);
+#else
+  return a * b;
+#endif
+}
+
+static inline ACO_HD uint32_t aco_sq15(int32_t d) { /* d^2 for |d| <= 32767 */
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (uint32_t)__mul24(d, d);
+#else
+  return (uint32_t)(d * d);
+#endif
+}
+
 static inline ACO_HD int32_t aco_nint_sqrt_u32(uint32_t s) {
   uint32_t r = (uint32_t)(aco_sqrt_estimate((float)s) + 0.5f);
-  const uint32_t sq = r * r;
+  const uint32_t sq = aco_umul24(r, r); /* r <= 46341: r^2 < 2^32 */
   if (s > sq + r) ++r;
   else if (r > 0u && s <= sq - r) --r;
   return (int32_t)r;
@@ -73,8 +92,7 @@ static inline ACO_HD int32_t aco_nint_sqrt_u32(uint32_t s) {
 
 /* TSPLIB EUC_2D distance for coordinates in [0, 32767]. */
 static inline ACO_HD int32_t aco_nint_dist(int32_t x1, int32_t y1, int32_t x2, int32_t y2) {
-  int32_t dx = x1 - x2, dy = y1 - y2;
-  uint32_t s = (uint32_t)(dx * dx) + (uint32_t)(dy * dy);
+  const uint32_t s = aco_sq15(x1 - x2) + aco_sq15(y1 - y2);
   return aco_nint_sqrt_u32(s);
 }
 

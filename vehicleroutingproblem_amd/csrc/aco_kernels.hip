@@ -662,6 +662,59 @@ __device__ __forceinline__ void fs_item(const DevCtx &P, const ColonyDesc &D, co
     f4[h] = make_float4(o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]);
 }
 
+/* The evaporating form of fs_item for the flat evaporation loop, with the
+ * item's pheromone group t already loaded: once this item's coordinate loads
+ * are issued, the group at `next` (when not null) is loaded into t_next for
+ * the thread's next item, so that load's latency overlaps this item's
+ * arithmetic instead of following it. */
+__device__ __forceinline__ void fs_item_evap_pf(const DevCtx &P, const ColonyDesc &D,
+                                                const InstDesc &I, const float *taupow,
+                                                const float *eta, uint32_t it, uint4 t,
+                                                uint32_t dq, const uint4 *next, uint4 &t_next) {
+  const int n1 = I.n + 1, S = I.fs_stride;
+  const uint32_t G = (uint32_t)S >> 4;
+  uint8_t *__restrict__ tau = P.pher + D.pher_off;
+  float *__restrict__ fs = P.fscore + D.fs_off;
+  const int2 *__restrict__ xy = P.xy + I.node_off;
+  const uint32_t u = it / G;
+  const int v0 = (int)(it - u * G) * 16;
+  const size_t row = (size_t)u * S + v0;
+  /* levels at 0 stay at 0: an all-zero group keeps its weights */
+  if ((t.x | t.y | t.z | t.w) == 0u) {
+    t_next = *next;
+    return;
+  }
+  t.x = evap_word(t.x, dq);
+  t.y = evap_word(t.y, dq);
+  t.z = evap_word(t.z, dq);
+  t.w = evap_word(t.w, dq);
+  *reinterpret_cast<uint4 *>(tau + row) = t;
+  const int2 pu = xy[u];
+  const int4 *x4 = reinterpret_cast<const int4 *>(xy + v0);
+  int4 q[8];
+#pragma unroll
+  for (int h = 0; h < 8; ++h) q[h] = x4[h];
+  t_next = *next;
+  const uint32_t tw[4] = {t.x, t.y, t.z, t.w};
+  float o[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int px = (i & 1) ? q[i >> 1].z : q[i >> 1].x;
+    const int py = (i & 1) ? q[i >> 1].w : q[i >> 1].y;
+    const uint32_t b = (tw[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+    /* past the last node the coordinates are the pool's padding (in
+     * bounds); the table reads are unconditional, at index 0 there */
+    const bool live = v0 + i < n1;
+    const int d = aco_nint_dist(pu.x, pu.y, px, py);
+    const float w = taupow[b] * eta[live ? d : 0];
+    o[i] = live ? w : 0.0f;
+  }
+  float4 *f4 = reinterpret_cast<float4 *>(fs + row);
+#pragma unroll
+  for (int h = 0; h < 4; ++h)
+    f4[h] = make_float4(o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]);
+}
+
 /* Items of the all-pairs weight matrix: item it covers row u = it / G and
  * the 16 columns 16g..16g+15 (G = S/16 items per row).  fscore[u][v] =
  * taupow[tau[u][v]] * eta[nint d(u,v)], 0 past the last node.  Pheromone and
@@ -787,6 +840,8 @@ __global__ void __launch_bounds__(512) k_evaporate_flat(DevCtx P, const long lon
   bool mat = false, ok = false;
   ColonyDesc D;
   InstDesc I;
+  uint4 tn = make_uint4(0u, 0u, 0u, 0u); /* pheromone group of item gn, loaded ahead */
+  long long gn = -1;
   for (long long g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
     if (g < lo || g >= hi) {
       mat = g < ti;
@@ -805,7 +860,19 @@ __global__ void __launch_bounds__(512) k_evaporate_flat(DevCtx P, const long lon
     const long long e = g - lo;
     if (mat) {
       if (P.fscore) {
-        fs_item(P, D, I, s_tp, eta, (uint32_t)e, true, d);
+        const uint4 *tb = reinterpret_cast<const uint4 *>(P.pher + D.pher_off);
+        uint4 t;
+        if (g == gn) t = tn;
+        else t = tb[e];
+        const long long g2 = g + blockDim.x;
+        const bool pf = g2 < g1 && g2 < hi;
+        /* always a load (of this item again when there is no next one), so
+         * that the wait counts are the same on every path */
+        const uint4 *nx = tb + (pf ? e + blockDim.x : e);
+        /* two copies, so that the LDS table is read as LDS */
+        if (eta_lds) fs_item_evap_pf(P, D, I, s_tp, s_eta, (uint32_t)e, t, d, nx, tn);
+        else fs_item_evap_pf(P, D, I, s_tp, P.eta, (uint32_t)e, t, d, nx, tn);
+        gn = pf ? g2 : -1;
       } else {
         uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
         uint4 v = base[e];
