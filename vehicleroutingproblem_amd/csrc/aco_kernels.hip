@@ -662,57 +662,51 @@ __device__ __forceinline__ void fs_item(const DevCtx &P, const ColonyDesc &D, co
     f4[h] = make_float4(o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]);
 }
 
-/* The evaporating form of fs_item for the flat evaporation loop, with the
- * item's pheromone group t already loaded: once this item's coordinate loads
- * are issued, the group at `next` (when not null) is loaded into t_next for
- * the thread's next item, so that load's latency overlaps this item's
- * arithmetic instead of following it. */
-__device__ __forceinline__ void fs_item_evap_pf(const DevCtx &P, const ColonyDesc &D,
-                                                const InstDesc &I, const float *taupow,
-                                                const float *eta, uint32_t it, uint4 t,
-                                                uint32_t dq, const uint4 *next, uint4 &t_next) {
+/* The evaporating form of fs_item for the flat evaporation loop, on one
+ * quad: the 4 columns v0..v0+3 (v0 = 16g + 4r) of row u of item `it`.
+ * Consecutive lanes take consecutive quads, so a wavefront's pheromone
+ * loads and stores and weight stores are contiguous (256 B and 1 KB per
+ * instruction).  The quad's pheromone word t is already loaded; once the
+ * coordinate loads are issued, the word at `next` is loaded into t_next for
+ * the thread's next quad, so its latency overlaps this quad's arithmetic. */
+__device__ __forceinline__ void fs_quad_evap(const DevCtx &P, const ColonyDesc &D,
+                                             const InstDesc &I, const float *taupow,
+                                             const float *eta, uint32_t it, uint32_t r,
+                                             uint32_t t, uint32_t dq, const uint32_t *next,
+                                             uint32_t &t_next) {
   const int n1 = I.n + 1, S = I.fs_stride;
   const uint32_t G = (uint32_t)S >> 4;
   uint8_t *__restrict__ tau = P.pher + D.pher_off;
   float *__restrict__ fs = P.fscore + D.fs_off;
   const int2 *__restrict__ xy = P.xy + I.node_off;
   const uint32_t u = it / G;
-  const int v0 = (int)(it - u * G) * 16;
+  const int v0 = (int)(it - u * G) * 16 + 4 * (int)r;
   const size_t row = (size_t)u * S + v0;
-  /* levels at 0 stay at 0: an all-zero group keeps its weights */
-  if ((t.x | t.y | t.z | t.w) == 0u) {
+  /* levels at 0 stay at 0: an all-zero quad keeps its weights */
+  if (t == 0u) {
     t_next = *next;
     return;
   }
-  t.x = evap_word(t.x, dq);
-  t.y = evap_word(t.y, dq);
-  t.z = evap_word(t.z, dq);
-  t.w = evap_word(t.w, dq);
-  *reinterpret_cast<uint4 *>(tau + row) = t;
+  t = evap_word(t, dq);
+  *reinterpret_cast<uint32_t *>(tau + row) = t;
   const int2 pu = xy[u];
   const int4 *x4 = reinterpret_cast<const int4 *>(xy + v0);
-  int4 q[8];
-#pragma unroll
-  for (int h = 0; h < 8; ++h) q[h] = x4[h];
+  const int4 qa = x4[0], qb = x4[1];
   t_next = *next;
-  const uint32_t tw[4] = {t.x, t.y, t.z, t.w};
-  float o[16];
+  const int px[4] = {qa.x, qa.z, qb.x, qb.z};
+  const int py[4] = {qa.y, qa.w, qb.y, qb.w};
+  float o[4];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int px = (i & 1) ? q[i >> 1].z : q[i >> 1].x;
-    const int py = (i & 1) ? q[i >> 1].w : q[i >> 1].y;
-    const uint32_t b = (tw[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+  for (int i = 0; i < 4; ++i) {
     /* past the last node the coordinates are the pool's padding (in
      * bounds); the table reads are unconditional, at index 0 there */
     const bool live = v0 + i < n1;
-    const int d = aco_nint_dist(pu.x, pu.y, px, py);
+    const int d = aco_nint_dist(pu.x, pu.y, px[i], py[i]);
+    const uint32_t b = (t >> (8 * i)) & 0xFFu;
     const float w = taupow[b] * eta[live ? d : 0];
     o[i] = live ? w : 0.0f;
   }
-  float4 *f4 = reinterpret_cast<float4 *>(fs + row);
-#pragma unroll
-  for (int h = 0; h < 4; ++h)
-    f4[h] = make_float4(o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]);
+  *reinterpret_cast<float4 *>(fs + row) = make_float4(o[0], o[1], o[2], o[3]);
 }
 
 /* Items of the all-pairs weight matrix: item it covers row u = it / G and
@@ -840,48 +834,62 @@ __global__ void __launch_bounds__(512) k_evaporate_flat(DevCtx P, const long lon
   bool mat = false, ok = false;
   ColonyDesc D;
   InstDesc I;
-  uint4 tn = make_uint4(0u, 0u, 0u, 0u); /* pheromone group of item gn, loaded ahead */
-  long long gn = -1;
-  for (long long g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
-    if (g < lo || g >= hi) {
-      mat = g < ti;
-      const long long *off = mat ? ioff : soff;
-      const long long base = mat ? 0 : ti;
-      const int ci = colony_of(off, ncol, g - base);
-      lo = base + off[ci];
-      hi = base + off[ci + 1];
-      const int c = cmap ? cmap[ci] : ci;
-      const AcoHipColonyState *S = P.state + c;
-      ok = !S->halted && P.feas[c] == S->epoch + 1;
-      D = P.col[c];
-      I = P.inst[D.inst];
+  /* cache the colony of flat index g: its range [lo, hi), whether it is in
+   * the matrix part, whether it evaporates this epoch, its descriptors */
+  auto locate = [&](long long g) __attribute__((always_inline)) {
+    if (g >= lo && g < hi) return;
+    mat = g < ti;
+    const long long *off = mat ? ioff : soff;
+    const long long base = mat ? 0 : ti;
+    const int ci = colony_of(off, ncol, g - base);
+    lo = base + off[ci];
+    hi = base + off[ci + 1];
+    const int c = cmap ? cmap[ci] : ci;
+    const AcoHipColonyState *S = P.state + c;
+    ok = !S->halted && P.feas[c] == S->epoch + 1;
+    D = P.col[c];
+    I = P.inst[D.inst];
+  };
+  long long gs = g0; /* first flat index of the generic loop below */
+  if (P.fscore) {
+    /* the matrix part of this block's range, quad by quad (4 quads per item) */
+    const long long q0 = 4 * (g0 < ti ? g0 : ti), q1 = 4 * (g1 < ti ? g1 : ti);
+    uint32_t tn = 0u; /* pheromone word of quad qn, loaded ahead */
+    long long qn = -1;
+    for (long long qq = q0 + threadIdx.x; qq < q1; qq += blockDim.x) {
+      const long long g = qq >> 2;
+      locate(g);
+      if (!ok) continue;
+      const uint32_t *tw = reinterpret_cast<const uint32_t *>(P.pher + D.pher_off);
+      const long long qe = qq - 4 * lo; /* quad within the colony */
+      uint32_t t;
+      if (qq == qn) t = tn;
+      else t = tw[qe];
+      const long long q2 = qq + blockDim.x;
+      const bool pf = q2 < q1 && (q2 >> 2) < hi;
+      /* always a load (of this quad again when there is no next one), so
+       * that the wait counts are the same on every path */
+      const uint32_t *nx = tw + (pf ? qe + blockDim.x : qe);
+      const uint32_t it = (uint32_t)(g - lo), r = (uint32_t)(qq & 3);
+      /* two copies, so that the LDS table is read as LDS */
+      if (eta_lds) fs_quad_evap(P, D, I, s_tp, s_eta, it, r, t, d, nx, tn);
+      else fs_quad_evap(P, D, I, s_tp, P.eta, it, r, t, d, nx, tn);
+      qn = pf ? q2 : -1;
     }
+    gs = g0 > ti ? g0 : ti;
+  }
+  for (long long g = gs + threadIdx.x; g < g1; g += blockDim.x) {
+    locate(g);
     if (!ok) continue;
     const long long e = g - lo;
     if (mat) {
-      if (P.fscore) {
-        const uint4 *tb = reinterpret_cast<const uint4 *>(P.pher + D.pher_off);
-        uint4 t;
-        if (g == gn) t = tn;
-        else t = tb[e];
-        const long long g2 = g + blockDim.x;
-        const bool pf = g2 < g1 && g2 < hi;
-        /* always a load (of this item again when there is no next one), so
-         * that the wait counts are the same on every path */
-        const uint4 *nx = tb + (pf ? e + blockDim.x : e);
-        /* two copies, so that the LDS table is read as LDS */
-        if (eta_lds) fs_item_evap_pf(P, D, I, s_tp, s_eta, (uint32_t)e, t, d, nx, tn);
-        else fs_item_evap_pf(P, D, I, s_tp, P.eta, (uint32_t)e, t, d, nx, tn);
-        gn = pf ? g2 : -1;
-      } else {
-        uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
-        uint4 v = base[e];
-        v.x = evap_word(v.x, d);
-        v.y = evap_word(v.y, d);
-        v.z = evap_word(v.z, d);
-        v.w = evap_word(v.w, d);
-        base[e] = v;
-      }
+      uint4 *base = reinterpret_cast<uint4 *>(P.pher + D.pher_off);
+      uint4 v = base[e];
+      v.x = evap_word(v.x, d);
+      v.y = evap_word(v.y, d);
+      v.z = evap_word(v.z, d);
+      v.w = evap_word(v.w, d);
+      base[e] = v;
     } else {
       uint8_t *tc = P.pher + D.tcand_off;
       uint32_t q = tc[e];
