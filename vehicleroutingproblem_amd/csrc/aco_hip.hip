@@ -614,8 +614,11 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.done = c->d_done;
 
 #define ACO_VARIANTS(R)                                                                         \
-  (const void *)k_construct<true, false, R>, (const void *)k_construct<false, false, R>,        \
-      (const void *)k_construct<true, true, R>, (const void *)k_construct<false, true, R>
+  (const void *)k_construct<true, false, R, false>,                                            \
+      (const void *)k_construct<false, false, R, false>,                                       \
+      (const void *)k_construct<true, true, R, false>,                                         \
+      (const void *)k_construct<false, true, R, false>,                                        \
+      (const void *)k_construct<true, false, R, true>, (const void *)k_construct<false, false, R, true>
   const void *variants[] = {ACO_VARIANTS(1), ACO_VARIANTS(2), ACO_VARIANTS(3), ACO_VARIANTS(4)};
 #undef ACO_VARIANTS
   for (const void *f : variants)
@@ -742,18 +745,27 @@ static void launch_construct_rows(AcoHipContext *c, const LaunchBucket &lb, hipS
   P.eta_lds = lb.eta_lds;
   const int *cols = c->d_colmap + lb.col_off;
   dim3 grid((unsigned)(lb.cols.size() * lb.groups)), block((unsigned)(64 * lb.W));
-  if (lb.stage && !stats)
-    hipLaunchKernelGGL((k_construct<true, false, R>), grid, block, lb.lds, st, P, lb.groups,
-                       lb.W, cols);
+  /* launches of several colonies keep each colony's workgroups on one XCD
+   * (the counter variants keep the plain order) */
+  const bool xcd = lb.cols.size() > 1;
+  if (lb.stage && !stats && xcd)
+    hipLaunchKernelGGL((k_construct<true, false, R, true>), grid, block, lb.lds, st, P,
+                       lb.groups, lb.W, cols);
+  else if (!stats && xcd)
+    hipLaunchKernelGGL((k_construct<false, false, R, true>), grid, block, lb.lds, st, P,
+                       lb.groups, lb.W, cols);
+  else if (lb.stage && !stats)
+    hipLaunchKernelGGL((k_construct<true, false, R, false>), grid, block, lb.lds, st, P,
+                       lb.groups, lb.W, cols);
   else if (!stats)
-    hipLaunchKernelGGL((k_construct<false, false, R>), grid, block, lb.lds, st, P, lb.groups,
-                       lb.W, cols);
+    hipLaunchKernelGGL((k_construct<false, false, R, false>), grid, block, lb.lds, st, P,
+                       lb.groups, lb.W, cols);
   else if (lb.stage)
-    hipLaunchKernelGGL((k_construct<true, true, R>), grid, block, lb.lds, st, P, lb.groups,
-                       lb.W, cols);
+    hipLaunchKernelGGL((k_construct<true, true, R, false>), grid, block, lb.lds, st, P,
+                       lb.groups, lb.W, cols);
   else
-    hipLaunchKernelGGL((k_construct<false, true, R>), grid, block, lb.lds, st, P, lb.groups,
-                       lb.W, cols);
+    hipLaunchKernelGGL((k_construct<false, true, R, false>), grid, block, lb.lds, st, P,
+                       lb.groups, lb.W, cols);
 }
 
 static void launch_bucket(AcoHipContext *c, const LaunchBucket &lb, hipStream_t st, bool stats) {

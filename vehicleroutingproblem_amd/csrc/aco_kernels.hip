@@ -349,14 +349,27 @@ __device__ __forceinline__ float cand_scan(float v, int lane) {
  * state (its demand, or VISITED); a DPP scan, two ballots and one bit-scan
  * select the move.  All control flow is uniform (scalar).  ROWS = 16-lane rows
  * the widest candidate list spans. */
-template <bool STAGE, bool STATS, int ROWS>
+template <bool STAGE, bool STATS, int ROWS, bool XCD>
 __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colony, int W,
                                                    const int *cols) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const unsigned long long t_kernel = STATS ? clock64() : 0ull;
   /* cols: the colonies of this launch (one launch per size bucket) */
-  const int slot = blockIdx.x / groups_per_colony;
-  const int group = blockIdx.x - slot * groups_per_colony;
+  /* Workgroups are dispatched to the 8 XCDs round-robin (workgroup i to
+   * XCD i mod 8), and each XCD has its own L2.  With XCD set (launches of
+   * many colonies) they are renumbered so that XCD x runs the x-th
+   * contiguous run of logical workgroups: the ants of a colony (consecutive
+   * logical workgroups) stay on one XCD, whose L2 then holds the candidate
+   * rows of an eighth of the colonies instead of all of them. */
+  unsigned lb = blockIdx.x;
+  if constexpr (XCD) {
+    const unsigned nxcd = 8;
+    const unsigned xq = gridDim.x / nxcd, xr = gridDim.x % nxcd;
+    const unsigned xid = blockIdx.x % nxcd, xi = blockIdx.x / nxcd;
+    lb = xid * xq + (xid < xr ? xid : xr) + xi;
+  }
+  const int slot = lb / groups_per_colony;
+  const int group = lb - slot * groups_per_colony;
   const int colony = cols ? cols[slot] : slot;
   const ColonyDesc D = P.col[colony];
   const AcoHipColonyState *S = P.state + colony;
@@ -588,11 +601,13 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   }
 }
 
-#define ACO_CONSTRUCT_VARIANTS(R)                                     \
-  template __global__ void k_construct<true, false, R>(DevCtx, int, int, const int *);  \
-  template __global__ void k_construct<false, false, R>(DevCtx, int, int, const int *); \
-  template __global__ void k_construct<true, true, R>(DevCtx, int, int, const int *);   \
-  template __global__ void k_construct<false, true, R>(DevCtx, int, int, const int *);
+#define ACO_CONSTRUCT_VARIANTS(R)                                                          \
+  template __global__ void k_construct<true, false, R, false>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct<false, false, R, false>(DevCtx, int, int, const int *); \
+  template __global__ void k_construct<true, true, R, false>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct<false, true, R, false>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct<true, false, R, true>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct<false, false, R, true>(DevCtx, int, int, const int *);
 ACO_CONSTRUCT_VARIANTS(1)
 ACO_CONSTRUCT_VARIANTS(2)
 ACO_CONSTRUCT_VARIANTS(3)
