@@ -201,6 +201,45 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
             "mfma_peak_tflops_bf16": 2500.0}
 
 
+def c2_seeds_leg(seeds: int = 32):
+    """The n=1000 workload over seeds 1..`seeds` (seed s sets the instance
+    and the solver seed), all in one context: once to the 80-epoch
+    stagnation stop (relative improvement 1e-3), each colony stopping on its
+    own, and once for a fixed 100 epochs.  Reports ant-tours/s over the
+    whole context, the epochs each colony ran, best costs and the fraction
+    of the last epoch's ants whose tour is feasible."""
+    from vehicleroutingproblem_amd import HipSolver, Instance, Params
+    insts = [Instance.uniform(1000, 8, side=1000, dmin=1, dmax=10, slack_pct=110, seed=s)
+             for s in range(1, seeds + 1)]
+    out = {"workload": f"{seeds} seeds of plain ACO n=1000 K=8 uniform, 256 ants, k=20, "
+                       "rho 0.1, in one context"}
+    for mode in ("stagnation_80", "fixed_100"):
+        with HipSolver(insts, Params(cand_k=CAND_K, ants=ANTS, rho=RHO),
+                       seeds=list(range(1, seeds + 1))) as s:
+            s.prepare()
+            s.sync()
+            t0 = time.perf_counter()
+            if mode == "fixed_100":
+                s.run(fixed_epochs=100, chunk_epochs=20)
+            else:
+                s.run(stagnation_epochs=80, max_epochs=5000, chunk_epochs=16)
+            s.sync()
+            dt = time.perf_counter() - t0
+            ep = [s.colony_state(i)["epoch"] for i in range(seeds)]
+            best = [s.colony_state(i)["global_best_cost"] for i in range(seeds)]
+            feas = []
+            for i in range(seeds):
+                _, _, _, unv = s.tours(i)
+                feas.append(float((unv == 0).mean()))
+        out[mode] = {"ant_tours_per_s": round(ANTS * sum(ep) / dt, 1),
+                     "seconds": round(dt, 3),
+                     "epochs_mean": round(sum(ep) / seeds, 1), "epochs_min": min(ep),
+                     "epochs_max": max(ep),
+                     "best_cost_mean": round(sum(best) / seeds, 1),
+                     "feasible_ant_fraction_last_epoch": round(sum(feas) / seeds, 4)}
+    return out
+
+
 def batched_leg(epochs: int, count: int = 512, ants: int = 128, fallback_matrix: int = -1):
     """C4: `count` instances in one context (n ~ U[50, 2000], half uniform,
     half clustered with sigma 30, K = ceil(n/125), demands U[1,10],
@@ -469,7 +508,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=64,
                     help="max epochs per graph replay (the largest divisor of --steps up to it)")
     ap.add_argument("--no-batch", action="store_true",
-                    help="skip the batched 512-instance and n=5000 legs")
+                    help="skip the 32-seed n=1000, batched 512-instance and n=5000 legs")
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the quality comparison with the reference sequential solver "
                          "(part of the CPU baselines, so --no-cpu skips it too)")
@@ -554,6 +593,7 @@ def main():
     epochs = st["epoch"] - args.warmup
     assert epochs == args.steps, (epochs, args.steps)
     value = ANTS * epochs / dt
+    feasible_last = float((s.tours(0)[3] == 0).mean())
 
     # Per-kernel durations over the same epochs as the timed region: a second
     # solver with the same instance and seed replays them bit for bit (the
@@ -596,6 +636,7 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * dt / epochs, 4),
         "epochs_per_s": round(epochs / dt, 1),
+        "feasible_ant_fraction_last_epoch": round(feasible_last, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -631,10 +672,11 @@ def main():
     }
     s.close()
     if not args.no_batch:
+        out["c2_seeds"] = c2_seeds_leg(32)
         out["batched"] = batched_leg(20)
         out["large_clustered"] = large_clustered_leg(20)
     if not args.no_prior:
-        out["learned_prior"] = learned_prior_leg(args.prior_seconds, 8, args.prior_train_count)
+        out["learned_prior"] = learned_prior_leg(args.prior_seconds, 32, args.prior_train_count)
     if not args.no_campaign:
         out["reference_campaign"] = reference_campaign_leg(2.0)
     if not args.no_cpu:
