@@ -46,7 +46,7 @@ enum {
   ACO_HIP_ERR_BACKEND = 4
 };
 
-/* Candidate-list ordering keys (SURVEY D3). */
+/* Candidate-list ordering keys. */
 enum {
   /* exact integer key (dx^2+dy^2, index): ties go to the lower index */
   ACO_HIP_KNN_EXACT = 0,
@@ -55,7 +55,7 @@ enum {
   ACO_HIP_KNN_REF_F32 = 1
 };
 
-/* Heuristic term used for candidate edges (north_star: learned prior). */
+/* Heuristic term used for candidate edges: (1/d)^beta or a learned prior. */
 enum {
   ACO_HIP_ETA_DISTANCE = 0, /* eta^beta = (1/d)^beta, reference aco_cuda_kernels.cu:207-212 */
   ACO_HIP_ETA_LEARNED = 1   /* per-candidate code set by aco_hip_set_candidate_eta() */

@@ -77,7 +77,7 @@ def test_scan_rows_order():
 def test_tau_tables_closed_forms():
     o = O.Oracle(Instance.uniform(20, 2, seed=1), cand_k=5, ants=4, rho=0.5)
     t = o.tables()
-    assert t["q_tau0"] == 170  # round(4/6 * 255) for tau0 = 1 (SURVEY closed form)
+    assert t["q_tau0"] == 170  # round(4/6 * 255) for tau0 = 1: (log 1 - log 1e-4) / (log 100 - log 1e-4) = 4/6
     assert t["q_evap"] == 13   # rho = 0.5
     o2 = O.Oracle(Instance.uniform(20, 2, seed=1), cand_k=5, ants=4, rho=0.1)
     assert o2.tables()["q_evap"] == 2
