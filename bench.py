@@ -531,7 +531,13 @@ def main():
                     help="only run the n=5000 clustered workload (1024 ants, k=32, one ant "
                          "per workgroup) for EPOCHS eager epochs (its kernels alone, for "
                          "counter runs)")
+    ap.add_argument("--batched-probe", type=int, default=0, metavar="EPOCHS",
+                    help="only run the batched 512-instance workload for EPOCHS timed epochs "
+                         "(its kernels alone, for profiler runs)")
     args = ap.parse_args()
+    if args.batched_probe:
+        print(json.dumps(batched_leg(args.batched_probe)))
+        return
     if args.large_probe:
         from vehicleroutingproblem_amd import HipSolver, Params
         inst = large_clustered_instance()
