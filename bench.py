@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE passes over this bench, each scaled by what the
 # counter reads for a 256 MiB device copy (fetch x2.0, write x1.0); the
 # commands and figures are in profiles/k_construct_n1000_m256_hbm_traffic.md.
-CONSTRUCT_TRAFFIC_BYTES = 20896682
+CONSTRUCT_TRAFFIC_BYTES = 20869168
 
 
 def tour_bytes(n, K, k):
