@@ -36,12 +36,13 @@ def tour_bytes(n, K, k):
     return (n + K) * (9 * k + 4)
 
 
-def cpu_baseline(inst, seconds_budget=12.0):
+def cpu_baseline(inst, seconds_budget=12.0, threads=0):
     """The CPU oracle (a C port of the same algorithm) on the host cores:
-    whole epochs of 256 ants, OpenMP over ants, for about `seconds_budget`."""
+    whole epochs of 256 ants, OpenMP over ants (`threads`, default all the
+    host's), for about `seconds_budget`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     o = O.Oracle(inst, cand_k=CAND_K, ants=ANTS, rho=RHO, seed=1)
     o.epochs(1, threads)  # warm caches
     t0 = time.perf_counter()
@@ -689,6 +690,7 @@ def main():
         # the CPU baselines: the port, the reference's OpenMP binary, and the
         # reference's sequential solver as the quality baseline at equal epochs
         out["cpu_baseline"] = cpu_baseline(inst)
+        out["cpu_baseline"]["port_single_thread"] = cpu_baseline(inst, 5.0, threads=1)
         ref = reference_openmp_baseline()
         if ref:
             out["cpu_baseline"]["reference_openmp"] = ref
