@@ -27,6 +27,11 @@ HBM_PEAK_GBS = 8000.0
 # counter reads for a 256 MiB device copy (fetch x2.0, write x1.0); the
 # commands and figures are in profiles/k_construct_n1000_m256_hbm_traffic.md.
 CONSTRUCT_TRAFFIC_BYTES = 20869168
+# Instructions k_construct issues per decision at this workload (VALU + SALU +
+# LDS + branch SQ counters over decisions, profiles/k_construct_n1000_m256_sq_counters.md):
+# a lone wavefront issues at most one instruction per 4-cycle slot of its SIMD,
+# so 4 x this is the issue bound of one decision in shader cycles.
+CONSTRUCT_INSTS_PER_DECISION = 103.0
 
 
 def tour_bytes(n, K, k):
@@ -672,7 +677,11 @@ def main():
             "cycles_per_decision": round(dev["ant_cycles"] / max(1, dev["decisions"]), 1),
             "cycles_per_ant": round(dev["ant_cycles"] / max(1, dev["ants"])),
             "staging_cycles_per_group": round(dev["staging_cycles"] * s.buckets()[0]["ants_per_group"]
-                                              / max(1, dev["ants"]))},
+                                              / max(1, dev["ants"])),
+            "issue_bound_cycles_per_decision": 4 * CONSTRUCT_INSTS_PER_DECISION,
+            "issue_bound_frac": round(4 * CONSTRUCT_INSTS_PER_DECISION
+                                      / max(1e-9, dev["ant_cycles"] / max(1, dev["decisions"])),
+                                      3)},
         "update_phases_us": ({
             k: round(1e6 * dev[k + "_ticks"] / max(1, dev["wall_clock_hz"]) / dev["updates"], 2)
             for k in ("evap", "select", "deposit")} if dev["updates"] else None),
