@@ -148,3 +148,13 @@ def test_cli_error_exits(argv, needle):
     r = subprocess.run([N.CLI_PATH] + argv, capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert needle in r.stderr
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU; asked for more it says so and exits 2
+    before touching a device."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
