@@ -149,11 +149,17 @@ def reference_sequential_baseline(seconds_budget=8.0):
 PRIOR_FLOPS_PER_EDGE_STEP = 2 * (8 * 64 + 64 * 64 + 64) + 2 * (64 * 64 * 2 + 64 * 8 + 64)
 
 
+# the edge-scoring network trains on the kNN-20 graph of its training
+# instances; the test solves with candidate lists of 16
+PRIOR_GRAPH_K = 20
+
+
 def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
     """Batched learned ACO against batched plain ACO at equal wall-clock on
     n=2000, K=16, 256 ants, k=16 instances (half uniform, half clustered):
-    the prior trains on elite tours of `train_count` seeded n in [200, 2000]
-    instances (harvested with plain ACO, 100 epochs, 64 ants), then the
+    the prior trains on the kNN-20 edges of `train_count` seeded n in
+    [200, 2000] instances, labelled by their elite tours (harvested with
+    plain ACO on 20-candidate lists, 100 epochs, 64 ants), then the
     `seeds` test instances run together in one batched context for `budget_s`
     seconds, once plain and once with the prior applied to every instance (the
     learned run's budget includes the prior's feature pass and inference)."""
@@ -161,7 +167,7 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
     from vehicleroutingproblem_amd.prior import train_prior, training_instances
     t0 = time.perf_counter()
     train = training_instances(train_count, seed=7)
-    prior = train_prior(train, epochs=100, steps=0, batch=512)
+    prior = train_prior(train, epochs=100, steps=0, batch=512, cand_k=PRIOR_GRAPH_K)
     t_harvest = time.perf_counter() - t0
     edges = prior.size()
     steps = 600
