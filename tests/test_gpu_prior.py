@@ -173,3 +173,35 @@ def test_learned_prior_gains_over_plain_aco(gpu):
     pr.close()
     print("learned-prior gain at equal wall-clock:", [round(100 * g, 2) for g in gains], "%")
     assert np.mean(gains) > 0.0, gains
+
+
+def test_prior_in_a_mixed_batch_matches_single_runs(gpu):
+    """A batched context of three instances of different size buckets where
+    only the middle one carries the learned heuristic: every instance runs
+    exactly as it does alone (with or without the prior), so the prior's
+    per-instance switch and the batched per-bucket chains do not interact."""
+    insts = [Instance.uniform(60, 2, seed=41), Instance.clustered(300, 4, side=1000, sigma=30,
+                                                                   seed=42),
+             Instance.uniform(700, 6, seed=43)]
+    seeds = [101, 102, 103]
+    p = Params(cand_k=12, ants=32, rho=0.1)
+    with EdgePrior(seed=4) as pr:
+        with HipSolver(insts, p, seeds=seeds) as b:
+            b.prepare()
+            pr.apply(b, instance=1, gain=2.0)
+            b.run(fixed_epochs=6, chunk_epochs=3)
+            got = [(b.tours(i), b.tau(i), b.colony_state(i)["global_best_cost"])
+                   for i in range(3)]
+        for i, inst in enumerate(insts):
+            with HipSolver(inst, p, seeds=[seeds[i]]) as s:
+                s.prepare()
+                if i == 1:
+                    pr.apply(s, instance=0, gain=2.0)
+                s.run(fixed_epochs=6, chunk_epochs=3)
+                (gt, gl, gc, gu), gtau, gbest = got[i]
+                st, sl, sc, su = s.tours(0)
+                assert np.array_equal(gl, sl) and np.array_equal(gc, sc), i
+                for a in range(len(gl)):
+                    assert np.array_equal(gt[a, :gl[a]], st[a, :sl[a]]), (i, a)
+                assert np.array_equal(gtau[0], s.tau(0)[0]), i
+                assert gbest == s.colony_state(0)["global_best_cost"], i
