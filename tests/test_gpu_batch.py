@@ -134,3 +134,27 @@ def test_bucket_chains_match_context_wide_updates(gpu, monkeypatch):
         assert a.colony_state(c) == b.colony_state(c)
     a.close()
     b.close()
+
+
+def test_counter_variants_match(gpu):
+    """The counter-collecting construct variants (plain workgroup order, no
+    XCD renumbering) build the same tours as the production variants of a
+    batched context, epoch after epoch, and report one decision per step of
+    every tour."""
+    insts, ks, seeds = _batch()
+    p = Params(cand_k=16, ants=24, rho=0.1, colonies=2)
+    out = []
+    for stats in (False, True):
+        with HipSolver(insts, p, seeds=seeds, cand_k=ks) as s:
+            s.prepare()
+            if stats:
+                s.set_stats(True)
+            s.run(fixed_epochs=4, chunk_epochs=2, use_graph=not stats)
+            out.append([s.tours(c) for c in range(len(insts) * 2)])
+            if stats:
+                st = s.stats(0)
+                assert st["ants"] == 4 * p.ants
+                assert st["decisions"] > 0
+    for c, (a, b) in enumerate(zip(out[0], out[1])):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), c
