@@ -339,3 +339,28 @@ def test_cli_smoke_instance(gpu, tmp_path):
     assert sum(1 for l in lines if l.startswith("Route ")) == 2
     assert any(l.startswith("Cost: ") for l in lines)
     assert any(l.startswith("best cost: ") for l in lines)
+
+
+def test_large_clustered_matches_oracle(gpu):
+    """The n=5000 clustered shape (k=32 rows read from global memory, the
+    fallback over five 1024-node chunks, a 25M-entry pheromone matrix): two
+    epochs of 8 ants equal the oracle's tours, states and pheromone."""
+    inst = Instance.clustered(5000, 40, side=10000, clusters=25, sigma=300, dmin=1, dmax=100,
+                              slack_pct=105, seed=3)
+    p = Params(cand_k=32, ants=8, rho=0.1)
+    with HipSolver(inst, p, seeds=[77]) as s:
+        assert s.stage_lds == 0
+        o = O.Oracle(inst, cand_k=32, ants=8, rho=0.1, seed=77)
+        s.prepare()
+        for e in range(2):
+            s.epochs(1)
+            o.epochs(1)
+            gt, gl, gc, gu = s.tours()
+            ot, ol, oc, ou = o.tours()
+            assert np.array_equal(gl, ol) and np.array_equal(gc, oc) and np.array_equal(gu, ou)
+            for a in range(len(gl)):
+                assert np.array_equal(gt[a, :gl[a]], ot[a, :ol[a]]), (e, a)
+            gs, os_ = s.colony_state(), o.state()
+            for key in ("epoch", "best_ant", "feasible_ants", "global_best_cost"):
+                assert gs[key] == os_[key], (e, key)
+            assert np.array_equal(s.tau()[0], o.tau()[0]), e
