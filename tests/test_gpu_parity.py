@@ -47,6 +47,12 @@ CASES = {
     # fallback, over three 1024-node chunks
     "n2100_k6_fallbacks": (lambda: Instance.uniform(2100, 12, seed=21),
                            dict(cand_k=6, ants=32, rho=0.1)),
+    # candidate lists over three and four 16-lane rows (the scan's readlane
+    # carries between rows 1-2 and 2-3; k = 64 fills the wavefront)
+    "k48_three_rows": (lambda: Instance.uniform(200, 4, seed=8),
+                       dict(cand_k=48, ants=32, rho=0.2)),
+    "k64_four_rows": (lambda: Instance.clustered(150, 3, side=1000, sigma=60, seed=9),
+                      dict(cand_k=64, ants=32, rho=0.2)),
 }
 
 
