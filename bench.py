@@ -3,10 +3,14 @@
 Workload (plain ACO at the reference's benchmark shape): n=1000 customers,
 K=8 vehicles, integer coordinates U{0..1000}^2, depot (500,500), demands
 U{1..10}, Q=ceil(1.1*sum(d)/8), 256 ants, candidate list 20, quantized
-pheromone, alpha=1, beta=2, rho=0.1.  A step is one epoch: every ant builds
-a tour, then best selection, evaporation and both deposits.  ant-tours/s =
-256 * epochs / wall seconds of the epoch loop (instance, kNN and pheromone
-already resident in HBM).
+pheromone, alpha=1, beta=2, rho=0.1, stagnation stop at 80 epochs, over
+seeds 1..32 (seed s sets the instance and the solver seed), the 32 seeds
+solved together in one context, one colony each.  A step is one epoch of
+every colony: every ant builds a tour, then best selection, evaporation and
+both deposits.  ant-tours/s = 256 * (epochs summed over the colonies) / wall
+seconds of the epoch loop (instances, kNN and pheromone already resident in
+HBM).  The single-seed figure (one colony of 256 ants) is the
+`single_seed` leg.
 
 Prints one JSON line (see the keys below).  --gpus N > 1 is not supported:
 this build runs on one GPU.
@@ -20,13 +24,16 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from vehicleroutingproblem_amd import workloads as W  # noqa: E402
+
 N_CUST, K_VEH, ANTS, CAND_K, RHO = 1000, 8, 256, 20, 0.1
 HBM_PEAK_GBS = 8000.0
-# HBM bytes one k_construct launch of this workload moves, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE passes over this bench, each scaled by what the
-# counter reads for a 256 MiB device copy (fetch x2.0, write x1.0); the
-# commands and figures are in profiles/k_construct_n1000_m256_hbm_traffic.md.
-CONSTRUCT_TRAFFIC_BYTES = 20869168
+# HBM bytes one k_construct launch of the n=1000 workload moves, by number of
+# seeds in the launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
+# this bench, each scaled by what the counter reads for a 256 MiB device copy
+# (fetch x2.0, write x1.0); the commands and figures are in
+# profiles/k_construct_n1000_m256_hbm_traffic.md.
+CONSTRUCT_TRAFFIC_BYTES = {1: 20869168}
 # Instructions k_construct issues per decision at this workload (VALU + SALU +
 # LDS + branch SQ counters over decisions, profiles/k_construct_n1000_m256_sq_counters.md):
 # a lone wavefront issues at most one instruction per 4-cycle slot of its SIMD,
@@ -56,92 +63,35 @@ def cpu_baseline(inst, seconds_budget=12.0, threads=0):
         o.epochs(1, threads)
         epochs += 1
     dt = time.perf_counter() - t0
+    o.close()
     return {"value": round(ANTS * epochs / dt, 1), "unit": "ant-tours/s", "cores": threads,
             "kind": "port",
-            "sample": f"{epochs} epochs x {ANTS} ants of the same n=1000 workload "
-                      f"({dt:.1f} s), oracle/aco_oracle.c with OpenMP over ants"}
+            "sample": f"{epochs} epochs x {ANTS} ants of the n=1000 workload (seed 1, "
+                      f"{dt:.1f} s), oracle/aco_oracle.c with OpenMP over ants"}
 
 
-def reference_openmp_baseline(seconds_budget=8.0, width20=False):
-    """The reference OpenMP CLI (built from its sources by oracle/ref.mk) on a
-    unit-demand n=1000, K=8 instance of the same generator (the reference
-    parser rejects weighted demand), 256 ants, ACO_SOLVER_FIXED_EPOCHS; its own
-    "Time:" line (epoch loop only).  Its candidate width comes from the host's
-    L3 size (k = 0.7 L3 / ((n+1) 8), clamped to [16, 512]); width20 runs the
-    same solver with the L3 probe answered "224K" (oracle/ref_omp_l3_driver.c),
-    which gives k = 20, this solver's width.  Reported beside the port; None
-    when the binary was not built."""
-    import re
-    import subprocess
-    import tempfile
-    from vehicleroutingproblem_amd import Instance
-    exe = os.path.join(ROOT, "oracle", "_ref", "aco_omp_l3" if width20 else "aco_omp")
-    if not os.path.exists(exe):
+def recorded_reference_baselines(port_value: float, port_single: float):
+    """The reference CLIs as timed in the build container, where their
+    sources are (tests/golden/ref_cpu_baselines.json, written by
+    tests/golden/make_golden.py; no reference binary runs on this host).
+    Each carries its speed relative to the port timed the same way on the
+    same cores there; `estimate_here` scales this host's port figure by that
+    ratio (an estimate: the two programs may scale differently across CPU
+    models)."""
+    path = os.path.join(ROOT, "tests", "golden", "ref_cpu_baselines.json")
+    if not os.path.exists(path):
         return None
-    inst = Instance.uniform(N_CUST, K_VEH, side=1000, dmin=1, dmax=1, slack_pct=120, seed=1)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    with tempfile.TemporaryDirectory() as td:
-        path = os.path.join(td, "unit_n1000_k8.vrp")
-        inst.write(path)
-
-        def run(epochs):
-            env = dict(os.environ, ACO_SOLVER_FIXED_EPOCHS=str(epochs),
-                       ACO_SOLVER_PROGRESS_INTERVAL_SECONDS="0", OMP_NUM_THREADS=str(threads))
-            if width20:
-                env["REF_L3_SIZE"] = "224K"
-            out = subprocess.run([exe, path, str(K_VEH), str(ANTS), "1"], env=env,
-                                 capture_output=True, text=True, timeout=300).stdout
-            m = re.search(r"Time: ([0-9.]+)s", out)
-            return float(m.group(1)) if m else None
-
-        t = run(4)
-        if not t:
-            return None
-        epochs = max(4, min(400, int(4 * seconds_budget / max(t, 1e-3))))
-        t = run(epochs)
-    if not t:
-        return None
-    return {"value": round(ANTS * epochs / t, 1), "unit": "ant-tours/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"reference src/openmp-mpi CLI, {epochs} fixed epochs x {ANTS} ants, "
-                      f"unit-demand n=1000 K=8 (real distances, rho=0.5, "
-                      + ("candidate width 20: L3 probe answered 224K" if width20 else
-                         "L3-tuned candidate width as shipped")
-                      + f"), {t:.2f} s solver time"}
-
-
-def reference_sequential_baseline(seconds_budget=8.0):
-    """The reference sequential CLI (aco_sequential.c, built from its sources
-    by oracle/ref.mk; candidate list = all n customers, one core) on the same
-    unit-demand n=1000, K=8 instance and 256 ants, stopped by its timeout:
-    epochs completed over the elapsed time of its last progress line.  None
-    when the binary was not built."""
-    import re
-    import subprocess
-    import tempfile
-    from vehicleroutingproblem_amd import Instance
-    exe = os.path.join(ROOT, "oracle", "_ref", "aco_seq")
-    if not os.path.exists(exe):
-        return None
-    inst = Instance.uniform(N_CUST, K_VEH, side=1000, dmin=1, dmax=1, slack_pct=120, seed=1)
-    with tempfile.TemporaryDirectory() as td:
-        path = os.path.join(td, "unit_n1000_k8.vrp")
-        inst.write(path)
-        env = dict(os.environ, ACO_SOLVER_TIMEOUT_SECONDS=str(seconds_budget),
-                   ACO_SOLVER_PROGRESS_INTERVAL_SECONDS="0.5")
-        out = subprocess.run([exe, path, str(K_VEH), str(ANTS), "1"], env=env,
-                             capture_output=True, text=True, timeout=300).stdout
-    prog = re.findall(r"epoca=([0-9]+) .*?tempo_trascorso=([0-9.]+)s", out)
-    if not prog:
-        return None
-    epochs, t = int(prog[-1][0]), float(prog[-1][1])
-    if t <= 0:
-        return None
-    return {"value": round(ANTS * epochs / t, 1), "unit": "ant-tours/s", "cores": 1,
-            "kind": "reference",
-            "sample": f"reference src/seq CLI, {epochs} epochs x {ANTS} ants in {t:.1f} s "
-                      "(its timeout stop), unit-demand n=1000 K=8, candidate list = all "
-                      "customers"}
+    doc = json.load(open(path))
+    out = {"measured_on": doc["host"], "port_there": doc["port"]["value"],
+           "port_single_thread_there": doc["port_single_thread"]["value"]}
+    for key in ("reference_openmp", "reference_openmp_width20"):
+        r = dict(doc[key])
+        r["estimate_here"] = round(port_value * r["over_port"], 1)
+        out[key] = r
+    r = dict(doc["reference_sequential"])
+    r["estimate_here"] = round(port_single * r["over_port_single_thread"], 1)
+    out["reference_sequential"] = r
+    return out
 
 
 # algorithmic flops per edge and training step: forward 2(8*64 + 64*64 + 64),
@@ -175,14 +125,7 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
     prior.train(steps, lr=2e-3)
     t_train = time.perf_counter() - t0
     flops = edges * steps * PRIOR_FLOPS_PER_EDGE_STEP
-    test = []
-    for t in range(seeds):
-        if t % 2 == 0:
-            test.append(Instance.uniform(2000, 16, side=1000, dmin=1, dmax=10, slack_pct=110,
-                                         seed=500 + t))
-        else:
-            test.append(Instance.clustered(2000, 16, side=1000, clusters=25, sigma=30, dmin=1,
-                                           dmax=10, slack_pct=110, seed=500 + t))
+    test = [W.c5_instance(t) for t in range(seeds)]
     res = {}
     for use in (False, True):
         with HipSolver(test, Params(cand_k=16, ants=256, rho=0.1),
@@ -213,64 +156,46 @@ def learned_prior_leg(budget_s: float, seeds: int, train_count: int):
             "mfma_peak_tflops_bf16": 2500.0}
 
 
-def c2_seeds_leg(seeds: int = 32):
-    """The n=1000 workload over seeds 1..`seeds` (seed s sets the instance
-    and the solver seed), all in one context: once to the 80-epoch
-    stagnation stop (relative improvement 1e-3), each colony stopping on its
-    own, and once for a fixed 100 epochs.  Reports ant-tours/s over the
-    whole context, the epochs each colony ran, best costs and the fraction
-    of the last epoch's ants whose tour is feasible."""
-    from vehicleroutingproblem_amd import HipSolver, Instance, Params
-    insts = [Instance.uniform(1000, 8, side=1000, dmin=1, dmax=10, slack_pct=110, seed=s)
-             for s in range(1, seeds + 1)]
-    out = {"workload": f"{seeds} seeds of plain ACO n=1000 K=8 uniform, 256 ants, k=20, "
-                       "rho 0.1, in one context"}
-    for mode in ("stagnation_80", "fixed_100"):
-        with HipSolver(insts, Params(cand_k=CAND_K, ants=ANTS, rho=RHO),
-                       seeds=list(range(1, seeds + 1))) as s:
-            s.prepare()
-            s.sync()
-            t0 = time.perf_counter()
-            if mode == "fixed_100":
-                s.run(fixed_epochs=100, chunk_epochs=20)
-            else:
-                s.run(stagnation_epochs=80, max_epochs=5000, chunk_epochs=16)
-            s.sync()
-            dt = time.perf_counter() - t0
-            ep = [s.colony_state(i)["epoch"] for i in range(seeds)]
-            best = [s.colony_state(i)["global_best_cost"] for i in range(seeds)]
-            feas = []
-            for i in range(seeds):
-                _, _, _, unv = s.tours(i)
-                feas.append(float((unv == 0).mean()))
-        out[mode] = {"ant_tours_per_s": round(ANTS * sum(ep) / dt, 1),
-                     "seconds": round(dt, 3),
-                     "epochs_mean": round(sum(ep) / seeds, 1), "epochs_min": min(ep),
-                     "epochs_max": max(ep),
-                     "best_cost_mean": round(sum(best) / seeds, 1),
-                     "feasible_ant_fraction_last_epoch": round(sum(feas) / seeds, 4)}
-    return out
+def c2_to_stagnation_leg(seeds: int = W.C2_SEEDS):
+    """The n=1000 workload over seeds 1..`seeds` in one context, every colony
+    run to its own 80-epoch stagnation stop (relative improvement 1e-3):
+    whole-run ant-tours/s, epochs per colony, best costs and the fraction of
+    the last epoch's ants whose tour is feasible."""
+    from vehicleroutingproblem_amd import HipSolver
+    sd = list(range(1, seeds + 1))
+    insts = [W.c2_instance(x) for x in sd]
+    with HipSolver(insts, W.c2_params(), seeds=sd) as s:
+        s.prepare()
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(stagnation_epochs=80, max_epochs=5000, chunk_epochs=16)
+        s.sync()
+        dt = time.perf_counter() - t0
+        ep = [s.colony_state(i)["epoch"] for i in range(seeds)]
+        best = [s.colony_state(i)["global_best_cost"] for i in range(seeds)]
+        feas = [float((s.tours(i)[3] == 0).mean()) for i in range(seeds)]
+    return {"workload": f"{seeds} seeds of the n=1000 workload in one context, each colony to "
+                        "its 80-epoch stagnation stop",
+            "ant_tours_per_s": round(W.C2_ANTS * sum(ep) / dt, 1), "seconds": round(dt, 3),
+            "epochs_mean": round(sum(ep) / seeds, 1), "epochs_min": min(ep),
+            "epochs_max": max(ep), "best_cost_mean": round(sum(best) / seeds, 1),
+            "feasible_ant_fraction_last_epoch": round(sum(feas) / seeds, 4)}
 
 
-def batched_leg(epochs: int, count: int = 512, ants: int = 128, fallback_matrix: int = -1):
-    """C4: `count` instances in one context (n ~ U[50, 2000], half uniform,
+def batched_leg(epochs: int, fallback_matrix: int = -1):
+    """C4: 512 instances in one context (n ~ U[50, 2000], half uniform,
     half clustered with sigma 30, K = ceil(n/125), demands U[1,10],
-    Q = ceil(1.1 sum/K)), `ants` ants each, per-instance seeds, candidate
+    Q = ceil(1.1 sum/K)), 128 ants each, per-instance seeds, candidate
     lengths from the tuned per-size-class table, launch shapes autotuned on
     the device; `epochs` timed epochs in graph replays after 2 warmup."""
-    import json as _json
-    from vehicleroutingproblem_amd import HipSolver, Params
-    from vehicleroutingproblem_amd.prior import training_instances
+    from vehicleroutingproblem_amd import HipSolver
     from vehicleroutingproblem_amd import tuner
-    insts = training_instances(count, nmin=50, nmax=2000, seed=4242)
-    table_path = os.path.join(ROOT, "vehicleroutingproblem_amd", "tuned_k.json")
-    k_table = {}
-    if os.path.exists(table_path):
-        k_table = {int(c): k for c, k in _json.load(open(table_path))["k_by_size_class"].items()}
-    ks = tuner.candidate_lengths(insts, k_table, default=16)
+    insts = W.c4_instances()
+    count = len(insts)
+    ks = W.c4_cand_k(insts)
     t0 = time.perf_counter()
-    s = HipSolver(insts, Params(cand_k=16, ants=ants, rho=0.1, fallback_matrix=fallback_matrix),
-                  seeds=[9000 + i for i in range(count)], cand_k=ks)
+    s = HipSolver(insts, W.c4_params(fallback_matrix=fallback_matrix), seeds=W.c4_seeds(),
+                  cand_k=ks)
     s.prepare()
     s.sync()
     t_prep = time.perf_counter() - t0
@@ -287,9 +212,10 @@ def batched_leg(epochs: int, count: int = 512, ants: int = 128, fallback_matrix:
     done = s.colony_state(0)["epoch"] - 2
     feasible = sum(1 for c in range(count) if s.colony_state(c)["global_best_cost"] < 2 ** 62)
     s.close()
+    k_table = W.tuned_k_table()
     return {"workload": f"{count} instances n~U[50,2000] (half uniform, half clustered sigma 30), "
-                        f"K=ceil(n/125), {ants} ants each",
-            "value": round(count * ants * done / dt, 1), "unit": "ant-tours/s",
+                        f"K=ceil(n/125), {W.C4_ANTS} ants each",
+            "value": round(count * W.C4_ANTS * done / dt, 1), "unit": "ant-tours/s",
             "epochs": done, "ms_per_epoch": round(1e3 * dt / max(done, 1), 3),
             "instances_with_feasible_tour": feasible,
             "cand_k_by_size_class": {str(c): k for c, k in sorted(k_table.items())} or "16",
@@ -299,68 +225,76 @@ def batched_leg(epochs: int, count: int = 512, ants: int = 128, fallback_matrix:
             "prepare_s": round(t_prep, 3), "autotune_s": round(t_tune, 3)}
 
 
-def large_clustered_instance():
-    """The n=5000 clustered workload: 25 Gaussian components, centres
-    U[0,10000]^2, sigma 300, depot at the centre, demands U[1,100], K=40,
-    Q=ceil(1.05 sum/40), seed 1."""
-    from vehicleroutingproblem_amd import Instance
-    return Instance.clustered(5000, 40, side=10000, clusters=25, sigma=300, dmin=1, dmax=100,
-                              slack_pct=105, seed=1)
+# HBM bytes one k_evaporate_flat launch of the n=5000 workload moves with the
+# all-pairs fallback weights on: rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled
+# by what the counter reads for a 256 MiB device copy (fetch x2.0, write x1.0);
+# commands and figures in profiles/k_evaporate_flat_n5000_m1024.md.
+C3_EVAP_TRAFFIC_BYTES = 152800000
 
 
 def large_clustered_leg(epochs: int = 20):
     """C3: n=5000 clustered (25 Gaussian components, centres U[0,10000]^2,
     sigma 300, depot at the centre), demands U[1,100], K=40,
-    Q=ceil(1.05 sum/40), 1024 ants, k=32; `epochs` timed epochs after 3."""
-    from vehicleroutingproblem_amd import HipSolver, Params
+    Q=ceil(1.05 sum/40), 1024 ants, k=32; `epochs` timed epochs after 3, with
+    the all-pairs fallback weight matrix on (the library's choice wherever it
+    fits) and off.  The evaporation's roofline counts its algorithmic bytes,
+    2(n+1)^2 (each pheromone level read and written once); the weight
+    matrix the kernel also rewrites is reported beside it as traffic."""
+    from vehicleroutingproblem_amd import HipSolver
     from vehicleroutingproblem_amd import tuner
-    inst = large_clustered_instance()
-    with HipSolver(inst, Params(cand_k=32, ants=1024, rho=0.1), seeds=[1]) as s:
-        s.prepare()
-        shapes = tuner.tune_shapes(s, cache=None, launches=4)
-        s.run(fixed_epochs=3, chunk_epochs=3)
-        s.sync()
-        t0 = time.perf_counter()
-        s.run(fixed_epochs=3 + epochs, chunk_epochs=div_chunk(epochs, 64))
-        s.sync()
-        dt = time.perf_counter() - t0
-        done = s.colony_state()["epoch"] - 3
-        best = s.colony_state()["global_best_cost"]
-        # per-kernel durations: 8 more epochs, eager launches with hipEvents
-        s.set_timing(True)
-        s.run(fixed_epochs=3 + epochs + 8, chunk_epochs=8, use_graph=False)
-        s.sync()
-        t = s.timing()
-        s.set_timing(False)
-        try:  # the all-pairs fallback weights exist iff the matrix is on
-            s.fallback_weights(0)
-            fs_on = True
-        except Exception:
-            fs_on = False
-    ep = max(1, t["construct_launches"])
-    evap_ms = t["evaporate_ms"] / max(1, t["evaporate_launches"])
-    # evaporation of the (n+1) x S pheromone matrix (S = n+1 rounded up to 16)
-    # in 16-byte items: each reads and writes its 16 levels and, with the
-    # all-pairs fallback weights on, writes their 16 fp32 weights
+    inst = W.c3_instance()
+    runs = {}
+    for fm in (1, 0):
+        with HipSolver(inst, W.c3_params(fallback_matrix=fm), seeds=[1]) as s:
+            s.prepare()
+            shapes = tuner.tune_shapes(s, cache=None, launches=4)
+            s.run(fixed_epochs=3, chunk_epochs=3)
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(fixed_epochs=3 + epochs, chunk_epochs=div_chunk(epochs, 64))
+            s.sync()
+            dt = time.perf_counter() - t0
+            done = s.colony_state()["epoch"] - 3
+            best = s.colony_state()["global_best_cost"]
+            # per-kernel durations: 8 more epochs, eager launches with hipEvents
+            s.set_timing(True)
+            s.run(fixed_epochs=3 + epochs + 8, chunk_epochs=8, use_graph=False)
+            s.sync()
+            t = s.timing()
+            s.set_timing(False)
+        ep = max(1, t["construct_launches"])
+        runs[fm] = {"value": round(W.C3_ANTS * done / dt, 1), "unit": "ant-tours/s",
+                    "ms_per_epoch": round(1e3 * dt / max(done, 1), 3), "epochs": done,
+                    "best_cost_after": best,
+                    "ants_per_group": shapes[0]["ants_per_group"],
+                    "stage_lds": shapes[0]["stage_lds"],
+                    "kernel_ms_per_epoch": {
+                        "construct": round(t["construct_ms"] / ep, 4),
+                        "evaporate": round(t["evaporate_ms"] / ep, 4),
+                        "select_and_deposit": round((t["best_ms"] + t["deposit_ms"]) / ep, 4)},
+                    "evaporate_avg_ms": t["evaporate_ms"] / max(1, t["evaporate_launches"])}
     n1 = inst.n + 1
-    items = n1 * ((n1 + 15) // 16)
-    evap_bytes = items * (32 + (64 if fs_on else 0))
-    evap_gbs = evap_bytes / (evap_ms * 1e-3) / 1e9
-    return {"workload": "n=5000 K=40 clustered (sigma 300 in [0,10000]^2), demands U[1,100], "
-                        "Q=ceil(1.05 sum/40), 1024 ants, k=32",
-            "value": round(1024 * done / dt, 1), "unit": "ant-tours/s",
-            "ms_per_epoch": round(1e3 * dt / max(done, 1), 3), "epochs": done,
-            "best_cost_after": best,
-            "ants_per_group": shapes[0]["ants_per_group"], "stage_lds": shapes[0]["stage_lds"],
-            "kernel_ms_per_epoch": {
-                "construct": round(t["construct_ms"] / ep, 4),
-                "evaporate": round(t["evaporate_ms"] / ep, 4),
-                "select_and_deposit": round((t["best_ms"] + t["deposit_ms"]) / ep, 4)},
-            "evaporation_roofline": {
-                "bound": "hbm", "kernel": "k_evaporate_flat", "achieved": round(evap_gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(evap_gbs / HBM_PEAK_GBS, 4),
-                "bytes_per_launch": evap_bytes, "avg_launch_ms": round(evap_ms, 4),
-                "pheromone_entries": n1 * n1, "fallback_matrix": bool(fs_on)}}
+    S = (n1 + 15) // 16 * 16
+    out = dict(runs[1])
+    evap_ms = out.pop("evaporate_avg_ms")
+    alg = 2 * n1 * n1
+    achieved = alg / (evap_ms * 1e-3) / 1e9
+    out["workload"] = ("n=5000 K=40 clustered (sigma 300 in [0,10000]^2), demands U[1,100], "
+                       "Q=ceil(1.05 sum/40), 1024 ants, k=32")
+    out["evaporation_roofline"] = {
+        "bound": "hbm", "kernel": "k_evaporate_flat", "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(evap_ms, 4),
+        "traffic": C3_EVAP_TRAFFIC_BYTES,
+        "weight_matrix_rewrite_bytes": 4 * n1 * S,
+        "note": "with the fallback weight matrix on, the kernel also rewrites taupow[q]*eta "
+                "for every pair (4 B each), about 2x the pheromone bytes"}
+    off = runs[0]
+    off_evap = off.pop("evaporate_avg_ms")
+    off["evaporate_achieved_gbs"] = round(alg / (off_evap * 1e-3) / 1e9, 1)
+    out["fallback_matrix_off"] = off
+    out["fallback_matrix_on_faster"] = runs[1]["ms_per_epoch"] <= runs[0]["ms_per_epoch"]
+    return out
 
 
 def real_tour_cost(inst, tour) -> float:
@@ -395,50 +329,40 @@ def tour_is_feasible(inst, tour) -> bool:
     return routes <= inst.K
 
 
-def quality_leg(seeds: int = 32, n: int = 100, ants: int = 64):
-    """CPU-baseline leg: solution quality against the reference's sequential solver at equal
-    epochs: unit-demand uniform instances (the reference parser's format),
-    n customers, K=8, Q=ceil(1.2 n/8); the reference sequential CLI (built
-    from its sources) runs to its stagnation stop (50 epochs) and reports
-    its epochs and best cost; this solver then runs exactly as many epochs
-    (alpha 1, beta 2, rho 0.5, tau0 1, Q 1, k=32, both CLIs' defaults) and its
-    best tour is costed in real (double) distances like the reference's."""
-    import re
-    import subprocess
-    import tempfile
-    import numpy as np
+def quality_leg():
+    """CPU-baseline leg: solution quality against the reference's sequential
+    solver at equal epochs.  The reference sequential CLI ran each of 32
+    seeded unit-demand instances (n=100, K=8, Q=ceil(1.2 n/8), 64 ants) to its
+    50-epoch stagnation stop in the build container; its best cost and epoch
+    count are the fixture tests/golden/ref_seq_quality.json.  This solver
+    runs exactly as many epochs with the same parameters (alpha 1, beta 2,
+    rho 0.5, tau0 1, Q 1, k=32) and its best tour is costed in real (double)
+    distances like the reference's print."""
     from vehicleroutingproblem_amd import HipSolver, Instance, Params
-    exe = os.path.join(ROOT, "oracle", "_ref", "aco_seq")
-    if not os.path.exists(exe):
+    path = os.path.join(ROOT, "tests", "golden", "ref_seq_quality.json")
+    if not os.path.exists(path):
         return None
+    doc = json.load(open(path))
+    runs = [r for r in doc["runs"] if r["set"] == "bench"]
     ref, ours, epochs = [], [], []
     feasible = True
-    with tempfile.TemporaryDirectory() as td:
-        for i in range(seeds):
-            inst = Instance.uniform(n, 8, side=1000, dmin=1, dmax=1, slack_pct=120, seed=700 + i)
-            path = os.path.join(td, f"u{i}.vrp")
-            inst.write(path)
-            env = dict(os.environ, ACO_SOLVER_STAGNATION_EPOCHS="50",
-                       ACO_SOLVER_TIMEOUT_SECONDS="120", ACO_SOLVER_PROGRESS_INTERVAL_SECONDS="0")
-            out = subprocess.run([exe, path, "8", str(ants), str(1234 + i)], env=env,
-                                 capture_output=True, text=True, timeout=300).stdout
-            m = re.search(r"best cost: ([0-9.]+)", out)
-            ep = [int(x) for x in re.findall(r"epoca=([0-9]+)", out)]
-            if not m or not ep:
-                return None
-            ref.append(float(m.group(1)))
-            epochs.append(max(ep))
-            with HipSolver(inst, Params(cand_k=32, ants=ants, alpha=1.0, beta=2.0, rho=0.5,
-                                        tau0=1.0, Q=1.0), seeds=[1234 + i]) as s:
-                s.prepare()
-                s.run(fixed_epochs=max(ep), chunk_epochs=div_chunk(max(ep), 64))
-                tour, _ = s.best()
-            feasible &= tour_is_feasible(inst, tour)
-            ours.append(real_tour_cost(inst, tour))
+    for r in runs:
+        inst = Instance.uniform(100, 8, side=1000, dmin=1, dmax=1, slack_pct=120,
+                                seed=r["instance_seed"])
+        with HipSolver(inst, Params(cand_k=32, ants=doc["ants"], alpha=1.0, beta=2.0, rho=0.5,
+                                    tau0=1.0, Q=1.0), seeds=[r["solver_seed"]]) as s:
+            s.prepare()
+            s.run(fixed_epochs=r["epochs"], chunk_epochs=div_chunk(r["epochs"], 64))
+            tour, _ = s.best()
+        feasible &= tour_is_feasible(inst, tour)
+        ours.append(real_tour_cost(inst, tour))
+        ref.append(r["best_cost"])
+        epochs.append(r["epochs"])
     diff = [(a - b) / b for a, b in zip(ours, ref)]
-    return {"workload": f"{seeds} unit-demand uniform instances, n={n}, K=8, Q=ceil(1.2n/8), "
-                        f"{ants} ants; reference sequential solver to its stagnation stop (50), "
-                        "this solver for the same number of epochs",
+    return {"workload": f"{len(runs)} unit-demand uniform instances, n=100, K=8, "
+                        f"Q=ceil(1.2n/8), {doc['ants']} ants; reference sequential solver to its "
+                        "stagnation stop (50), recorded in the build container; this solver for "
+                        "the same number of epochs",
             "mean_best_cost_diff_pct": round(100 * sum(diff) / len(diff), 3),
             "all_tours_feasible": feasible,
             "mean_epochs": round(sum(epochs) / len(epochs), 1),
@@ -510,17 +434,115 @@ def div_chunk(epochs: int, cap: int) -> int:
     return max([c for c in range(1, min(cap, max(epochs, 1)) + 1) if epochs % c == 0] or [1])
 
 
+def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int = 0) -> dict:
+    """The n=1000 workload (256 ants, k=20, rho 0.1, stagnation stop at 80)
+    for the given seeds in one context (one colony per seed): `warmup`
+    epochs, then exactly `steps` timed epochs of the solver loop in graph
+    replays.  Per-kernel durations come from a second context that replays
+    the same epochs bit for bit with hipEvents around every launch (eager
+    launches on the context stream), device counters from a third."""
+    from vehicleroutingproblem_amd import HipSolver
+    seeds = list(seeds)
+    insts = [W.c2_instance(x) for x in seeds]
+    p = W.c2_params(ants_per_group=ants_per_group)
+    chunk = div_chunk(steps, chunk_cap)
+
+    def fresh():
+        r_ = HipSolver(insts, p, seeds=seeds)
+        r_.prepare()
+        if warmup > 0:
+            r_.run(stagnation_epochs=80, max_epochs=warmup,
+                   chunk_epochs=div_chunk(warmup, chunk_cap), use_graph=True)
+        r_.sync()
+        return r_
+
+    s = fresh()
+    t0 = time.perf_counter()
+    s.run(stagnation_epochs=80, max_epochs=warmup + steps, chunk_epochs=chunk, use_graph=True)
+    s.sync()
+    dt = time.perf_counter() - t0
+    states = [s.colony_state(i) for i in range(len(seeds))]
+    epochs = [st["epoch"] - warmup for st in states]
+    feasible_last = sum(float((s.tours(i)[3] == 0).mean()) for i in range(len(seeds))) / len(seeds)
+    shape = (s.ants_per_group, s.stage_lds)
+    s.close()
+
+    def replay(timing: bool):
+        r_ = fresh()
+        if timing:
+            r_.set_timing(True)
+        else:
+            r_.set_stats(True)
+        r_.run(stagnation_epochs=80, max_epochs=warmup + steps, chunk_epochs=chunk,
+               use_graph=not timing)
+        r_.sync()
+        for i, st in enumerate(states):
+            assert r_.colony_state(i)["global_best_cost"] == st["global_best_cost"]
+        if timing:
+            out_ = r_.timing()
+        else:
+            out_ = {}
+            for i in range(len(seeds)):
+                for k_, v_ in r_.stats(i).items():
+                    out_[k_] = v_ if k_ == "wall_clock_hz" else out_.get(k_, 0) + v_
+        r_.close()
+        return out_
+
+    t = replay(True)
+    assert t["construct_launches"] == max(epochs), t
+    dev = replay(False)
+    launches = max(1, t["construct_launches"])
+    construct_ms = t["construct_ms"] / launches
+    bytes_per_launch = len(seeds) * W.C2_ANTS * tour_bytes(W.C2_N, W.C2_K, W.C2_CAND_K)
+    achieved = bytes_per_launch / (construct_ms * 1e-3) / 1e9
+    cpd = dev["ant_cycles"] / max(1, dev["decisions"])
+    return {
+        "value": round(W.C2_ANTS * sum(epochs) / dt, 1),
+        "ms_per_step": round(1e3 * dt / max(epochs), 4),
+        "epochs_per_s": round(max(epochs) / dt, 1),
+        "epochs_timed": max(epochs),
+        "feasible_ant_fraction_last_epoch": round(feasible_last, 4),
+        "best_cost": [st["global_best_cost"] for st in states][:8],
+        "ants_per_group": shape[0], "stage_lds": shape[1],
+        "roofline": {"bound": "hbm", "kernel": "k_construct", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": CONSTRUCT_TRAFFIC_BYTES.get(len(seeds)),
+                     "avg_launch_ms": round(construct_ms, 5),
+                     "bytes_per_launch": bytes_per_launch},
+        "kernel_ms_per_epoch": {
+            "construct": round(construct_ms, 5),
+            "evaporate": round(t["evaporate_ms"] / launches, 5),
+            "select_and_deposit": round((t["best_ms"] + t["deposit_ms"]) / launches, 5)},
+        "construct_counters": {
+            "decisions_per_tour": round(dev["decisions"] / max(1, dev["ants"]), 1),
+            "fallback_rate": round(dev["fallbacks"] / max(1, dev["decisions"]), 4),
+            "fallback_cycle_share": round(dev["fallback_cycles"] / max(1, dev["ant_cycles"]), 4),
+            "cycles_per_decision": round(cpd, 1),
+            "cycles_per_ant": round(dev["ant_cycles"] / max(1, dev["ants"])),
+            "issue_bound_cycles_per_decision": 4 * CONSTRUCT_INSTS_PER_DECISION,
+            "issue_bound_frac": round(4 * CONSTRUCT_INSTS_PER_DECISION / max(1e-9, cpd), 3)},
+        "update_phases_us": ({
+            k: round(1e6 * dev[k + "_ticks"] / max(1, dev["wall_clock_hz"]) / dev["updates"], 2)
+            for k in ("evap", "select", "deposit")} if dev["updates"] else None),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--seeds", type=int, default=W.C2_SEEDS,
+                    help="seeds of the n=1000 workload solved together (one colony each)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ants-per-group", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=64,
                     help="max epochs per graph replay (the largest divisor of --steps up to it)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-seed n=1000 leg")
     ap.add_argument("--no-batch", action="store_true",
-                    help="skip the 32-seed n=1000, batched 512-instance and n=5000 legs")
+                    help="skip the stagnation-stop, batched 512-instance and n=5000 legs")
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the quality comparison with the reference sequential solver "
                          "(part of the CPU baselines, so --no-cpu skips it too)")
@@ -546,15 +568,27 @@ def main():
     ap.add_argument("--batched-probe", type=int, default=0, metavar="EPOCHS",
                     help="only run the batched 512-instance workload for EPOCHS timed epochs "
                          "(its kernels alone, for profiler runs)")
+    ap.add_argument("--headline-probe", type=int, default=0, metavar="EPOCHS",
+                    help="only run the n=1000 workload over --seeds seeds for EPOCHS eager "
+                         "epochs (its kernels alone, for counter runs)")
     args = ap.parse_args()
+    if args.headline_probe:
+        from vehicleroutingproblem_amd import HipSolver
+        sd = list(range(1, args.seeds + 1))
+        with HipSolver([W.c2_instance(x) for x in sd], W.c2_params(), seeds=sd) as s:
+            s.prepare()
+            s.run(fixed_epochs=args.headline_probe, chunk_epochs=args.headline_probe,
+                  use_graph=False)
+            s.sync()
+            print(json.dumps({"headline_probe_epochs": s.colony_state()["epoch"],
+                              "seeds": len(sd)}))
+        return
     if args.batched_probe:
         print(json.dumps(batched_leg(args.batched_probe)))
         return
     if args.large_probe:
-        from vehicleroutingproblem_amd import HipSolver, Params
-        inst = large_clustered_instance()
-        with HipSolver(inst, Params(cand_k=32, ants=1024, rho=0.1, ants_per_group=1),
-                       seeds=[1]) as s:
+        from vehicleroutingproblem_amd import HipSolver
+        with HipSolver(W.c3_instance(), W.c3_params(ants_per_group=1), seeds=[1]) as s:
             s.prepare()
             s.run(fixed_epochs=args.large_probe, chunk_epochs=args.large_probe, use_graph=False)
             s.sync()
@@ -587,114 +621,39 @@ def main():
               "sharding of this path is a design note, not a built path.", file=sys.stderr)
         sys.exit(2)
 
-    from vehicleroutingproblem_amd import HipSolver, Instance, Params
-
-    inst = Instance.uniform(N_CUST, K_VEH, side=1000, dmin=1, dmax=10, slack_pct=110, seed=1)
-    p = Params(cand_k=CAND_K, ants=ANTS, rho=RHO, ants_per_group=args.ants_per_group)
-    s = HipSolver(inst, p, seeds=[1])
-    s.prepare()
-    s.sync()
-    # warmup (also captures the epoch graph)
-    if args.warmup > 0:
-        s.run(fixed_epochs=args.warmup, chunk_epochs=div_chunk(args.warmup, args.chunk),
-              use_graph=True)
-    s.sync()
-    # timed region: exactly `steps` epochs of the solver loop, in graph
-    # replays of `chunk` epochs (a divisor of steps, so no replay runs past
-    # the last epoch)
-    chunk = div_chunk(args.steps, args.chunk)
-    t0 = time.perf_counter()
-    r = s.run(fixed_epochs=args.warmup + args.steps, chunk_epochs=chunk, use_graph=True)
-    s.sync()
-    dt = time.perf_counter() - t0
-    st = s.colony_state()
-    epochs = st["epoch"] - args.warmup
-    assert epochs == args.steps, (epochs, args.steps)
-    value = ANTS * epochs / dt
-    feasible_last = float((s.tours(0)[3] == 0).mean())
-
-    # Per-kernel durations over the same epochs as the timed region: a second
-    # solver with the same instance and seed replays them bit for bit (the
-    # path is deterministic), with hipEvents on its stream around every
-    # launch (eager launches).  A third replay collects the device counters.
-    def replay(timing: bool, stats: bool):
-        r_ = HipSolver(inst, p, seeds=[1])
-        r_.prepare()
-        if args.warmup > 0:
-            r_.run(fixed_epochs=args.warmup, chunk_epochs=div_chunk(args.warmup, args.chunk),
-                   use_graph=True)
-        r_.sync()
-        if timing:
-            r_.set_timing(True)
-        if stats:
-            r_.set_stats(True)
-        r_.run(fixed_epochs=args.warmup + args.steps, chunk_epochs=chunk,
-               use_graph=not timing)
-        r_.sync()
-        assert r_.colony_state()["global_best_cost"] == st["global_best_cost"]
-        out_ = r_.timing() if timing else r_.stats()
-        r_.close()
-        return out_
-
-    t = replay(True, False)
-    assert t["construct_launches"] == args.steps, t
-    dev = replay(False, True)
-    construct_ms = t["construct_ms"] / max(1, t["construct_launches"])
-    evap_ms = t["evaporate_ms"] / max(1, t["evaporate_launches"])
-    k = min(CAND_K, inst.n)
-    bytes_per_launch = ANTS * tour_bytes(inst.n, inst.K, k)
-    achieved = bytes_per_launch / (construct_ms * 1e-3) / 1e9
-
+    seeds = list(range(1, args.seeds + 1))
+    head = c2_run(seeds, args.steps, args.warmup, args.chunk, args.ants_per_group)
     out = {
         "metric": "ant-tours/s",
-        "value": round(value, 1),
+        "value": head.pop("value"),
         "unit": "ant-tours/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * dt / epochs, 4),
-        "epochs_per_s": round(epochs / dt, 1),
-        "feasible_ant_fraction_last_epoch": round(feasible_last, 4),
+        "ms_per_step": head.pop("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+u8",
         "data": "synthetic",
-        "config": {"workload": "plain ACO n=1000 K=8 uniform [0,1000]^2 demands U[1,10] "
-                               "Q=ceil(1.1*sum/8), 256 ants, k=20, quantized pheromone",
-                   "n": inst.n, "K": inst.K, "ants": ANTS, "cand_k": k, "rho": RHO,
-                   "ants_per_group": s.ants_per_group, "stage_lds": s.stage_lds,
-                   "best_cost": st["global_best_cost"]},
-        "roofline": {"bound": "hbm", "kernel": "k_construct", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": CONSTRUCT_TRAFFIC_BYTES,
-                     "avg_launch_ms": round(construct_ms, 5),
-                     "bytes_per_launch": bytes_per_launch,
-                     "evaporate_avg_ms": round(evap_ms, 5)},
-        "kernel_ms_per_epoch": {
-            "construct": round(t["construct_ms"] / max(1, t["construct_launches"]), 5),
-            "evaporate": round(t["evaporate_ms"] / max(1, t["construct_launches"]), 5),
-            "select_and_deposit": round((t["best_ms"] + t["deposit_ms"])
-                                        / max(1, t["construct_launches"]), 5)},
-        "construct_counters": {
-            "decisions_per_tour": round(dev["decisions"] / max(1, dev["ants"]), 1),
-            "fallback_rate": round(dev["fallbacks"] / max(1, dev["decisions"]), 4),
-            "fallback_cycle_share": round(dev["fallback_cycles"] / max(1, dev["ant_cycles"]), 4),
-            "cycles_per_decision": round(dev["ant_cycles"] / max(1, dev["decisions"]), 1),
-            "cycles_per_ant": round(dev["ant_cycles"] / max(1, dev["ants"])),
-            "staging_cycles_per_group": round(dev["staging_cycles"] * s.buckets()[0]["ants_per_group"]
-                                              / max(1, dev["ants"])),
-            "issue_bound_cycles_per_decision": 4 * CONSTRUCT_INSTS_PER_DECISION,
-            "issue_bound_frac": round(4 * CONSTRUCT_INSTS_PER_DECISION
-                                      / max(1e-9, dev["ant_cycles"] / max(1, dev["decisions"])),
-                                      3)},
-        "update_phases_us": ({
-            k: round(1e6 * dev[k + "_ticks"] / max(1, dev["wall_clock_hz"]) / dev["updates"], 2)
-            for k in ("evap", "select", "deposit")} if dev["updates"] else None),
+        "config": {"workload": f"plain ACO n=1000 K=8 uniform [0,1000]^2 demands U[1,10] "
+                               f"Q=ceil(1.1*sum/8), 256 ants, k=20, rho 0.1, quantized "
+                               f"pheromone, stagnation stop 80; seeds 1..{len(seeds)} (seed s "
+                               f"sets instance and solver) solved together, one colony each; "
+                               f"a step is one epoch of every colony",
+                   "n": W.C2_N, "K": W.C2_K, "ants": W.C2_ANTS, "cand_k": W.C2_CAND_K,
+                   "rho": W.C2_RHO, "seeds": len(seeds),
+                   "ants_per_group": head.pop("ants_per_group"),
+                   "stage_lds": head.pop("stage_lds")},
+        "roofline": head.pop("roofline"),
     }
-    s.close()
+    out.update(head)
+    if not args.no_single and len(seeds) > 1:
+        single = c2_run([1], max(args.steps, 20), args.warmup, args.chunk, args.ants_per_group)
+        single["workload"] = "the same n=1000 workload, seed 1 only (one colony of 256 ants)"
+        out["single_seed"] = single
     if not args.no_batch:
-        out["c2_seeds"] = c2_seeds_leg(32)
+        out["c2_to_stagnation"] = c2_to_stagnation_leg(len(seeds))
         out["batched"] = batched_leg(20)
         out["large_clustered"] = large_clustered_leg(20)
     if not args.no_prior:
@@ -702,21 +661,17 @@ def main():
     if not args.no_campaign:
         out["reference_campaign"] = reference_campaign_leg(2.0)
     if not args.no_cpu:
-        # the CPU baselines: the port, the reference's OpenMP binary, and the
-        # reference's sequential solver as the quality baseline at equal epochs
+        # the CPU baseline: the port on this host's cores (the reference's
+        # own CLIs were timed in the build container, beside the port)
+        inst = W.c2_instance(1)
         out["cpu_baseline"] = cpu_baseline(inst)
-        out["cpu_baseline"]["port_single_thread"] = cpu_baseline(inst, 5.0, threads=1)
-        ref = reference_openmp_baseline()
-        if ref:
-            out["cpu_baseline"]["reference_openmp"] = ref
-        ref20 = reference_openmp_baseline(width20=True)
-        if ref20:
-            out["cpu_baseline"]["reference_openmp_width20"] = ref20
-        seq = reference_sequential_baseline()
-        if seq:
-            out["cpu_baseline"]["reference_sequential"] = seq
+        single = cpu_baseline(inst, 5.0, threads=1)
+        out["cpu_baseline"]["port_single_thread"] = single
+        rec = recorded_reference_baselines(out["cpu_baseline"]["value"], single["value"])
+        if rec:
+            out["cpu_baseline"]["reference_recorded"] = rec
         if not args.no_quality:
-            out["cpu_baseline"]["quality_vs_reference_seq"] = quality_leg(32)
+            out["cpu_baseline"]["quality_vs_reference_seq"] = quality_leg()
     print(json.dumps(out))
 
 

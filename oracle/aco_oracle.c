@@ -546,6 +546,20 @@ static int significant(double prev, double now, double min_rel) {
   return (prev - now) / fmax(prev, eps) + eps >= min_rel;
 }
 
+int oracle_significant(double prev, double now, double min_rel) {
+  return significant(prev, now, min_rel);
+}
+
+/* Replaces the ants' results of the current epoch (costs, unvisited counts,
+ * tour lengths; any may be NULL) so that the best selection and stop rules
+ * can be driven with chosen values. */
+void oracle_set_ant_results(OracleColony *o, const int64_t *cost, const int32_t *unvisited,
+                            const int32_t *len) {
+  if (cost) memcpy(o->cost, cost, sizeof(int64_t) * (size_t)o->m);
+  if (unvisited) memcpy(o->unvisited, unvisited, sizeof(int32_t) * (size_t)o->m);
+  if (len) memcpy(o->len, len, sizeof(int32_t) * (size_t)o->m);
+}
+
 void oracle_select_best(OracleColony *o) {
   OracleState *st = &o->st;
   if (st->halted) {

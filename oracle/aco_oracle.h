@@ -46,6 +46,11 @@ void oracle_construct(OracleColony *o, int nthreads);
 void oracle_get_tours(const OracleColony *o, int32_t *tours, int32_t *len, int64_t *cost,
                       int32_t *unvisited);
 void oracle_select_best(OracleColony *o);
+/* is_significant_improvement (src/cuda/aco_cuda.cu:83-94). */
+int oracle_significant(double prev, double now, double min_rel);
+/* Test hook: replace the current epoch's ant results (NULL keeps a field). */
+void oracle_set_ant_results(OracleColony *o, const int64_t *cost, const int32_t *unvisited,
+                            const int32_t *len);
 void oracle_evaporate(OracleColony *o);
 void oracle_deposit_tour(OracleColony *o, const int32_t *tour, int len, float delta);
 void oracle_deposit_iter(OracleColony *o);

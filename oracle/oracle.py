@@ -47,6 +47,8 @@ _SIG = {
     "oracle_construct": (None, [C.c_void_p, C.c_int]),
     "oracle_get_tours": (None, [C.c_void_p, i32p, i32p, i64p, i32p]),
     "oracle_select_best": (None, [C.c_void_p]),
+    "oracle_significant": (C.c_int, [C.c_double, C.c_double, C.c_double]),
+    "oracle_set_ant_results": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "oracle_evaporate": (None, [C.c_void_p]),
     "oracle_deposit_tour": (None, [C.c_void_p, i32p, C.c_int, C.c_float]),
     "oracle_deposit_iter": (None, [C.c_void_p]),
@@ -186,6 +188,21 @@ class Oracle:
     def select_best(self):
         self.lib.oracle_select_best(self.h)
 
+    def set_ant_results(self, cost=None, unvisited=None, lens=None):
+        """Replace this epoch's per-ant costs / unvisited counts / lengths."""
+        keep = []
+
+        def arr(a, t):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, t)
+            assert a.shape == (self.m,)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        self.lib.oracle_set_ant_results(self.h, arr(cost, np.int64), arr(unvisited, np.int32),
+                                        arr(lens, np.int32))
+
     def evaporate(self):
         self.lib.oracle_evaporate(self.h)
 
@@ -241,6 +258,10 @@ def scan_rows(w, nlanes):
     out = np.zeros(64, np.float32)
     lib.oracle_scan_rows(_p(w, C.c_float), nlanes, _p(out, C.c_float))
     return out
+
+
+def significant(prev, now, min_rel):
+    return bool(load().oracle_significant(prev, now, min_rel))
 
 
 def nint_dist(x1, y1, x2, y2):

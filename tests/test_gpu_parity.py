@@ -112,6 +112,21 @@ def test_candidates_reference_key(gpu, name):
     s.close()
 
 
+@pytest.mark.parametrize("name", ["n500_k8_s19000", "n1000_k8_s19001"])
+@pytest.mark.parametrize("k", [16, 20, 32])
+def test_candidates_equal_reference_golden(gpu, name, k):
+    """k_knn with the reference key on the reference's own benchmark
+    instances gives exactly the lists the reference's aco_mpi_shared_init
+    builds (src/openmp-mpi/aco_parallel.c:218-250; golden lists in
+    tests/golden, written by make_golden.py from the reference sources)."""
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    inst = Instance.read(os.path.join(golden, f"ref_{name}.vrp"), require_unit_demand=True)
+    gold = np.load(os.path.join(golden, f"ref_knn_{name}.npz"))[f"k{k}"]
+    with HipSolver(inst, Params(cand_k=k, ants=4, knn_key=N.KNN_REF_F32)) as s:
+        s.build_candidates()
+        assert np.array_equal(s.candidates()[0], gold)
+
+
 def _compare_tours(s, o):
     gt, gl, gc, gu = s.tours()
     ot, ol, oc, ou = o.tours()

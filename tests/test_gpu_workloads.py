@@ -1,0 +1,195 @@
+"""GPU parity at the benchmark's own workloads: the exact instances, seeds,
+launch shapes and epoch-loop path (hipGraph chunks) that bench.py times,
+compared with the CPU oracle bit for bit.
+
+* c2: n=1000, K=8, 256 ants, k=20, seed 1, LDS-staged rows with 4 ants per
+  workgroup (the headline shape), 3 epochs; and the 32-seed context (one
+  colony per seed) against per-seed oracle runs on a sample of colonies.
+* c3: n=5000 clustered, 1024 ants, k=32, one ant per workgroup, candidate
+  rows read from global memory, 2 epochs.
+* c4: the 512-instance batch (n ~ U[50,2000]) with the tuned candidate
+  lengths and autotuned launch shapes, 2 epochs in one graph replay: 16
+  colonies spanning every size bucket against oracle runs, and every tour of
+  all 512 colonies checked for route structure, capacity and exact cost.
+The reference order being restated is src/cuda/aco_cuda.cu:331-429.
+"""
+import numpy as np
+import pytest
+
+from vehicleroutingproblem_amd import HipSolver
+from vehicleroutingproblem_amd import workloads as W
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STATE_KEYS = ("epoch", "iter_since_best", "halted", "found", "best_ant", "improved", "best_len",
+              "feasible_ants", "iter_best_cost", "global_best_cost")
+
+
+def oracle_for(inst, p, seed, colony=0, cand_k=None):
+    return O.Oracle(inst, cand_k=cand_k or p.cand_k, ants=p.ants, alpha=p.alpha, beta=p.beta,
+                    rho=p.rho, tau0=p.tau0, Q=p.Q, depot_w=p.depot_close_weight,
+                    knn_key=p.knn_key, seed=seed, colony=colony)
+
+
+def nint_matrix(inst):
+    """nint(d(u, v)) for every pair: the square root of an integer is never
+    within 1e-6 of a half-integer, so rounding the double root is exact."""
+    x = np.asarray(inst.x, np.int64)
+    y = np.asarray(inst.y, np.int64)
+    s = (x[:, None] - x[None, :]) ** 2 + (y[:, None] - y[None, :]) ** 2
+    return np.rint(np.sqrt(s.astype(np.float64))).astype(np.int64)
+
+
+def compare_colony(s, o, colony, inst, scores=True, weights=True, tag=""):
+    """Tours, lengths, costs, unvisited counts, colony state, pheromone,
+    candidate-aligned pheromone, candidate scores and (when on) the all-pairs
+    fallback weights of GPU colony `colony` against oracle colony `o`."""
+    gt, gl, gc, gu = s.tours(colony)
+    ot, ol, oc, ou = o.tours()
+    assert np.array_equal(gl, ol), tag
+    assert np.array_equal(gc, oc), tag
+    assert np.array_equal(gu, ou), tag
+    for a in range(len(gl)):
+        assert np.array_equal(gt[a, :gl[a]], ot[a, :ol[a]]), (tag, "ant", a)
+    gs, os_ = s.colony_state(colony), o.state()
+    for key in STATE_KEYS:
+        assert gs[key] == os_[key], (tag, key, gs[key], os_[key])
+    assert np.float32(gs["dep_iter"]) == np.float32(os_["dep_iter"]), tag
+    assert np.float32(gs["dep_best"]) == np.float32(os_["dep_best"]), tag
+    tg, cg = s.tau(colony)
+    to, co = o.tau()
+    assert np.array_equal(tg, to), (tag, "tau")
+    assert np.array_equal(cg, co), (tag, "tau_cand")
+    t = o.tables()
+    if scores:
+        _, dist, _ = o.candidates()
+        want = (t["taupow"][co] * t["eta"][dist]).astype(np.float32)
+        assert np.array_equal(s.scores(colony).view(np.uint32), want.view(np.uint32)), tag
+    if weights:
+        want = (t["taupow"][to] * t["eta"][nint_matrix(inst)]).astype(np.float32)
+        got = s.fallback_weights(colony)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (tag, "weights")
+    if os_["global_best_cost"] < 2 ** 62:
+        bt, bc = s.best(colony)
+        ot_, oc_ = o.best()
+        assert bc == oc_ and np.array_equal(bt, ot_), tag
+
+
+def check_all_tours(inst, tours, lens, cost, unvis):
+    """Every ant's tour: starts and ends at the depot, K+1 depot entries,
+    customers distinct and n - unvisited of them, every route's load <= Q,
+    cost = the exact nint length."""
+    m, stride = tours.shape
+    pos = np.arange(stride)[None, :]
+    inside = pos < lens[:, None]
+    t = np.where(inside, tours, 0).astype(np.int64)
+    assert (t[:, 0] == 0).all()
+    assert (t[np.arange(m), lens - 1] == 0).all()
+    assert ((t == 0) & inside).sum(1).tolist() == [inst.K + 1] * m
+    cust = (t > 0) & inside
+    assert (cust.sum(1) == inst.n - unvis).all()
+    # distinct customers: other positions get distinct keys above n, so a
+    # repeated value in a sorted row can only be a repeated customer
+    srt = np.sort(np.where(cust, t, inst.n + 1 + pos), axis=1)
+    assert (np.diff(srt, axis=1) != 0).all()
+    # loads per route: route id = number of depot entries before the position
+    rid = np.cumsum((t == 0) & inside, axis=1)
+    dem = np.asarray(inst.demand, np.int64)[t] * cust
+    loads = np.zeros((m, inst.K + 2), np.int64)
+    np.add.at(loads, (np.repeat(np.arange(m), stride), rid.reshape(-1)), dem.reshape(-1))
+    assert loads.max() <= inst.capacity
+    x = np.asarray(inst.x, np.int64)[t]
+    y = np.asarray(inst.y, np.int64)[t]
+    s = (x[:, 1:] - x[:, :-1]) ** 2 + (y[:, 1:] - y[:, :-1]) ** 2
+    d = np.rint(np.sqrt(s.astype(np.float64))).astype(np.int64)
+    d = np.where(pos[:, 1:] < lens[:, None], d, 0)
+    assert np.array_equal(d.sum(1), cost)
+
+
+def test_c2_headline_shape_matches_oracle(gpu):
+    inst = W.c2_instance(1)
+    p = W.c2_params()
+    with HipSolver(inst, p, seeds=[1]) as s:
+        # the shape the benchmark runs: LDS-staged rows, 4 ants per workgroup
+        assert s.ants_per_group == 4 and s.stage_lds == 1
+        o = oracle_for(inst, p, 1)
+        s.prepare()
+        s.run(fixed_epochs=1, chunk_epochs=1, use_graph=True)
+        o.epochs(1)
+        compare_colony(s, o, 0, inst, tag="epoch 1")
+        s.run(fixed_epochs=3, chunk_epochs=2, use_graph=True)
+        o.epochs(2)
+        compare_colony(s, o, 0, inst, tag="epoch 3")
+        check_all_tours(inst, *s.tours(0))
+
+
+def test_c2_thirty_two_seeds_in_one_context(gpu):
+    seeds = list(range(1, W.C2_SEEDS + 1))
+    insts = [W.c2_instance(sd) for sd in seeds]
+    p = W.c2_params()
+    with HipSolver(insts, p, seeds=seeds) as s:
+        s.prepare()
+        s.run(fixed_epochs=2, chunk_epochs=2, use_graph=True)
+        for c in range(len(seeds)):
+            check_all_tours(insts[c], *s.tours(c))
+        for c in (0, 7, 16, 31):
+            o = oracle_for(insts[c], p, seeds[c])
+            o.epochs(2)
+            compare_colony(s, o, c, insts[c], tag=f"seed {seeds[c]}")
+            o.close()
+
+
+def test_c3_large_clustered_matches_oracle(gpu):
+    inst = W.c3_instance()
+    p = W.c3_params(ants_per_group=1, stage_lds=0)
+    with HipSolver(inst, p, seeds=[1]) as s:
+        assert s.ants_per_group == 1 and s.stage_lds == 0
+        o = oracle_for(inst, p, 1)
+        s.prepare()
+        s.run(fixed_epochs=1, chunk_epochs=1, use_graph=True)
+        o.epochs(1)
+        compare_colony(s, o, 0, inst, weights=False, tag="epoch 1")
+        s.run(fixed_epochs=2, chunk_epochs=1, use_graph=True)
+        o.epochs(1)
+        compare_colony(s, o, 0, inst, weights=True, tag="epoch 2")
+        check_all_tours(inst, *s.tours(0))
+
+
+def test_c4_batched_matches_oracle(gpu):
+    from vehicleroutingproblem_amd import tuner
+    insts = W.c4_instances()
+    ks = W.c4_cand_k(insts)
+    seeds = W.c4_seeds()
+    p = W.c4_params()
+    with HipSolver(insts, p, seeds=seeds, cand_k=ks) as s:
+        s.prepare()
+        shapes = tuner.tune_shapes(s, cache=None, launches=1)
+        assert len(shapes) == 4
+        s.run(fixed_epochs=2, chunk_epochs=2, use_graph=True)
+        for c, inst in enumerate(insts):
+            tours, lens, cost, unvis = s.tours(c)
+            check_all_tours(inst, tours, lens, cost, unvis)
+            st = s.colony_state(c)
+            assert st["epoch"] == 2
+            feasible = unvis == 0
+            assert st["feasible_ants"] == int(feasible.sum())
+            if feasible.any():
+                assert st["iter_best_cost"] == cost[feasible].min()
+                bt, bc = s.best(c)
+                check_all_tours(inst, bt[None, :], np.array([len(bt)], np.int32),
+                                np.array([bc], np.int64), np.zeros(1, np.int64))
+        # four colonies of every size bucket against oracle runs of their instances
+        by_bucket = {}
+        for c, inst in enumerate(insts):
+            by_bucket.setdefault(tuner.size_class(inst.n + 1), []).append(c)
+        assert sorted(by_bucket) == [256, 512, 1024, 2048]
+        sample = []
+        for cls in sorted(by_bucket):
+            cs = by_bucket[cls]
+            sample += [cs[0], cs[1], cs[-2], cs[-1]]
+        for c in sample:
+            o = oracle_for(insts[c], p, seeds[c], cand_k=ks[c])
+            o.epochs(2)
+            compare_colony(s, o, c, insts[c], tag=f"colony {c} n={insts[c].n} k={ks[c]}")
+            o.close()

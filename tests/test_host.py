@@ -158,3 +158,21 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
+
+
+def test_quality_fixture_instances_regenerate():
+    """The instances the reference sequential CLI solved for the quality
+    fixture (tests/golden/ref_seq_quality.json) are regenerated bit for bit
+    by the seeded generator, so the GPU quality test solves the same ones."""
+    import hashlib
+    import json
+    doc = json.load(open(os.path.join(GOLDEN, "ref_seq_quality.json")))
+    assert len(doc["runs"]) == 40
+    for r in doc["runs"]:
+        inst = Instance.uniform(100, 8, side=1000, dmin=1, dmax=1, slack_pct=120,
+                                seed=r["instance_seed"])
+        h = hashlib.sha1()
+        for a in (inst.x, inst.y, inst.demand):
+            h.update(np.ascontiguousarray(a, np.int32).tobytes())
+        assert h.hexdigest() == r["instance_sha1"], r["instance_seed"]
+        assert r["epochs"] >= 50 and r["best_cost"] > 0

@@ -57,3 +57,30 @@ def test_labels_from_tour():
                      [0, 0, 0],    # 4: unvisited
                      [0, 0, 0]])
     assert np.array_equal(y, want)
+
+
+def test_features_hand_computed_rows():
+    """A 3-node instance worked by hand: depot (10,20), customer 1 (13,24)
+    with demand 3, customer 2 (10,30) with demand 5, Q = 8, k = 2, so the
+    bounding-box scale is L = max(13-10, 30-20) = 10.  Candidate rows:
+    depot -> [1, 2] (nint 5, 10), 1 -> [2, pad 0] (nint 7, 0),
+    2 -> [1, pad 0] (nint 7, 0).  Expected features (before BF16 rounding):
+      slot (1,0), j=2: (0.3, 0.4, 0, 1, 5/8, 0.7, cos, 0) with cos of the
+        angle at 1 between (-3,6) and (-3,-4) = -15/(sqrt(45)*5) = -1/sqrt(5);
+      slot (0,1), j=2: (0, 0, 0, 1, 5/8, 1.0, 0 (i is the depot), 1/2);
+      slot (2,0), j=1: (0, 1, 0.3, 0.4, 3/8, 0.7, 60/(sqrt(45)*10) = 2/sqrt(5), 0)."""
+    x, y, dem, Q, k = [10, 13, 10], [20, 24, 30], [0, 3, 5], 8, 2
+    cand = [[1, 2], [2, 0], [1, 0]]
+    dist = [[5, 10], [7, 0], [7, 0]]
+    f = M.bf16_float(M.features(x, y, dem, Q, cand, dist, k)).reshape(3, k, M.F)
+    r5 = 5.0 ** 0.5
+    want = {(1, 0): [0.3, 0.4, 0.0, 1.0, 0.625, 0.7, -1.0 / r5, 0.0],
+            (0, 1): [0.0, 0.0, 0.0, 1.0, 0.625, 1.0, 0.0, 0.5],
+            (2, 0): [0.0, 1.0, 0.3, 0.4, 0.375, 0.7, 2.0 / r5, 0.0]}
+    for (i, t), w in want.items():
+        w = np.array(w)
+        got = f[i, t].astype(np.float64)
+        # BF16 keeps 8 significant bits: round to nearest is within 2^-8 relative
+        assert np.all(np.abs(got - w) <= np.abs(w) * 2.0 ** -8), (i, t, got, w)
+        exact = np.isin(w, [0.0, 0.375, 0.5, 0.625, 1.0])
+        assert np.array_equal(got[exact], w[exact]), (i, t)
