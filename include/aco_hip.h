@@ -249,6 +249,19 @@ int aco_hip_deposit_tour(AcoHipContext *ctx, int colony, const int32_t *tour, in
  * the reference accumulates it in-kernel (aco_cuda_kernels.cu:540,550). */
 int aco_hip_route_costs(AcoHipContext *ctx, int instance, const int32_t *tours,
                         const int32_t *len, int count, int stride, int64_t *cost);
+/* Route costs without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as aco_hip_route_costs, but the tours, lengths and
+ * costs are staged in a fixed device array of ACO_HIP_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds until the process ends, instead of buffers allocated
+ * per call.  A batch needing more staging than that (count*stride*4 +
+ * count*4 bytes, each rounded up to 256, + count*8) is refused with
+ * ACO_HIP_ERR_INVALID_INPUT.  All calls on one GPU share its array: calls
+ * that overlap on one GPU (several host threads) race.  Synchronous. */
+#define ACO_HIP_NOALLOC_SCRATCH_BYTES (4u << 20)
+int aco_hip_route_costs_noalloc(AcoHipContext *ctx, int instance, const int32_t *tours,
+                                const int32_t *len, int count, int stride, int64_t *cost);
 /* Host copies of the shared tables: deq[256], thr[255], taupow[256], eta[eta_len]. */
 int aco_hip_get_tables(AcoHipContext *ctx, float *deq, float *thr, float *taupow,
                        float *eta_lut, int *eta_len, uint8_t *q_tau0, uint8_t *q_evap);

@@ -132,6 +132,8 @@ HIP_SYMBOLS = {
     "aco_hip_get_best": (C.c_int, [ctx_p, C.c_int, i32p, i32p, i64p]),
     "aco_hip_deposit_tour": (C.c_int, [ctx_p, C.c_int, i32p, C.c_int, C.c_float]),
     "aco_hip_route_costs": (C.c_int, [ctx_p, C.c_int, i32p, i32p, C.c_int, C.c_int, i64p]),
+    "aco_hip_route_costs_noalloc": (C.c_int, [ctx_p, C.c_int, i32p, i32p, C.c_int, C.c_int,
+                                              i64p]),
     "aco_hip_get_tables": (C.c_int, [ctx_p, f32p, f32p, f32p, f32p, i32p, u8p, u8p]),
     "aco_hip_set_stats": (C.c_int, [ctx_p, C.c_int]),
     "aco_hip_get_stats": (C.c_int, [ctx_p, C.c_int, C.c_void_p]),

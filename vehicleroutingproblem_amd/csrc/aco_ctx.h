@@ -109,6 +109,12 @@ int launch_refresh(AcoHipContext *c);
 /* Switch instance `inst` to (learned != 0) or from the per-slot heuristic
  * already in d_ceta, and refresh the scores (stream-ordered, synchronous). */
 int ctx_set_learned(AcoHipContext *c, int inst, int learned);
+/* Argument checks shared by the route-cost entry points (0 = OK). */
+int check_route_cost_args(AcoHipContext *c, int inst, const int32_t *tours, const int32_t *len,
+                          int count, int stride, const int64_t *cost);
+/* Looks up the device address of the allocation-free entry point's staging
+ * array for `device` (current); 0 on success. */
+int noalloc_attach_device(int device);
 
 #define HIPC(ctx, call)                                                                         \
   do {                                                                                          \

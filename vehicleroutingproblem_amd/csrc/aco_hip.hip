@@ -182,6 +182,9 @@ extern "C" int aco_hip_create(AcoHipContext **out, int device) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
     c->num_cu = cus;
+  /* device address of the staging array of the allocation-free entry point
+   * (looked up once per device; a failure only disables that entry point) */
+  (void)noalloc_attach_device(device);
   *out = c;
   return ACO_HIP_OK;
 }
@@ -1375,9 +1378,8 @@ extern "C" int aco_hip_deposit_tour(AcoHipContext *c, int colony, const int32_t 
   return ACO_HIP_OK;
 }
 
-extern "C" int aco_hip_route_costs(AcoHipContext *c, int inst, const int32_t *tours,
-                                   const int32_t *len, int count, int stride, int64_t *cost) {
-  NEED_LOADED(c);
+int check_route_cost_args(AcoHipContext *c, int inst, const int32_t *tours, const int32_t *len,
+                          int count, int stride, const int64_t *cost) {
   int rc;
   if ((rc = check_inst(c, inst))) return rc;
   if (count <= 0) return ACO_HIP_OK;
@@ -1391,8 +1393,17 @@ extern "C" int aco_hip_route_costs(AcoHipContext *c, int inst, const int32_t *to
       if (tours[(size_t)t * stride + p] < 0 || tours[(size_t)t * stride + p] > n)
         return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tour %d node out of range", t);
   }
+  return ACO_HIP_OK;
+}
+
+extern "C" int aco_hip_route_costs(AcoHipContext *c, int inst, const int32_t *tours,
+                                   const int32_t *len, int count, int stride, int64_t *cost) {
+  NEED_LOADED(c);
+  int rc = check_route_cost_args(c, inst, tours, len, count, stride, cost);
+  if (rc || count <= 0) return rc;
   int32_t *dt = nullptr, *dl = nullptr;
   long long *dc = nullptr;
+  HIPC(c, hipSetDevice(c->device));
   HIPC(c, hipMalloc(&dt, sizeof(int32_t) * (size_t)count * stride));
   HIPC(c, hipMalloc(&dl, sizeof(int32_t) * count));
   HIPC(c, hipMalloc(&dc, sizeof(long long) * count));

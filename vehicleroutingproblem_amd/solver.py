@@ -374,6 +374,17 @@ class HipSolver:
                                                  tours.shape[1], N.ptr(out, C.c_int64)))
         return out
 
+    def route_costs_noalloc(self, tours, lens, instance: int = 0) -> np.ndarray:
+        """route_costs staged in the library's fixed device array (no device
+        allocation; batches above ACO_HIP_NOALLOC_SCRATCH_BYTES are refused)."""
+        tours = np.ascontiguousarray(tours, dtype=np.int32)
+        lens = np.ascontiguousarray(lens, dtype=np.int32)
+        out = np.zeros(tours.shape[0], np.int64)
+        self._check(self.lib.aco_hip_route_costs_noalloc(
+            self._ctx, instance, N.ptr(tours, C.c_int32), N.ptr(lens, C.c_int32), tours.shape[0],
+            tours.shape[1], N.ptr(out, C.c_int64)))
+        return out
+
     def tables(self) -> dict:
         deq = np.zeros(256, np.float32)
         thr = np.zeros(255, np.float32)
