@@ -18,7 +18,13 @@
  *
  * Data conventions (identical on host, device and in the CPU oracle):
  *   - nodes 0..n, node 0 is the depot; integer coordinates in [0, 32767];
- *   - distances are TSPLIB EUC_2D nearest-integer: nint(sqrt(dx^2+dy^2));
+ *   - distances (AcoHipParams.dist_mode): TSPLIB EUC_2D nearest-integer
+ *     nint(sqrt(dx^2+dy^2)) (ACO_HIP_DIST_NINT), or the reference's unrounded
+ *     distance as the float nearest to sqrt(dx^2+dy^2) (ACO_HIP_DIST_REAL,
+ *     src/common/instance_parser.c:17-19, src/cuda/aco_cuda_kernels.cu:53-57);
+ *   - costs are int64 in cost units: the nint length, or in real mode the
+ *     exact sum of the float edge lengths times 2^23 (real cost = cost *
+ *     ACO_HIP_REAL_COST_UNIT);
  *   - pheromone is the reference's log-quantized uint8 format
  *     (src/cuda/aco_cuda_kernels.cu:25-45, src/cuda/aco_cuda.cu:209-242):
  *     an (n+1)x(n+1) byte matrix with row stride n+1;
@@ -54,6 +60,13 @@ enum {
    * '<' insertion, ties to the lower index (src/openmp-mpi/aco_parallel.c:218-250) */
   ACO_HIP_KNN_REF_F32 = 1
 };
+
+/* Distance of an edge, the objective and the heuristic 1/d. */
+enum {
+  ACO_HIP_DIST_NINT = 0, /* TSPLIB EUC_2D nearest integer, integer costs */
+  ACO_HIP_DIST_REAL = 1  /* unrounded (float nearest to the root), the reference's metric */
+};
+#define ACO_HIP_REAL_COST_UNIT (1.0 / 8388608.0)
 
 /* Heuristic term used for candidate edges: (1/d)^beta or a learned prior. */
 enum {
@@ -94,6 +107,8 @@ typedef struct {
   int fallback_matrix;   /* -1 auto, 0 off, 1 on: keep the all-pairs fallback weights
                             taupow[tau[u][v]]*eta[d(u,v)] as a float matrix per colony
                             ((n+1) x roundup(n+1,16) floats) instead of recomputing them */
+  int dist_mode;         /* ACO_HIP_DIST_*: real mode optimises unrounded lengths, with
+                            (1/d)^beta taken from a table binned at relative width 2^-10 */
 } AcoHipParams;
 
 /* Device-side state of one colony, read back with aco_hip_get_colony(). */
@@ -212,8 +227,10 @@ int aco_hip_epochs(AcoHipContext *ctx, int count);
 int aco_hip_run(AcoHipContext *ctx, const AcoHipRunConfig *cfg, AcoHipRunStats *stats);
 
 /* ---- downloads / uploads (synchronous) ---------------------------------- */
-/* (n+1)*k candidate indices, nint distances and heuristic values (any may be
- * NULL): eta^beta of the distance, or the learned heuristic once one is set. */
+/* (n+1)*k candidate indices, distance codes and heuristic values (any may be
+ * NULL).  The code is the nint distance, or in real mode the heuristic-table
+ * index of the float distance (aco_real_eta_bin); the heuristic is eta^beta
+ * of the distance, or the learned heuristic once one is set. */
 int aco_hip_get_candidates(AcoHipContext *ctx, int instance, int32_t *idx, int32_t *dist,
                            float *eta);
 /* Learned heuristic prior (the seam of aco_cuda_kernels.cu:295, where eta_beta
@@ -334,15 +351,17 @@ int aco_hip_prior_apply(AcoHipPrior *prior, AcoHipContext *ctx, int instance, fl
 int aco_hip_set_timing(AcoHipContext *ctx, int enable);
 int aco_hip_get_timing(AcoHipContext *ctx, AcoHipTiming *t);
 
-/* ---- one-call solver (drop-in for aco_vrp_cuda_with_capacity,
- *      src/cuda/main_vrp.cu:29-35): one instance, one colony.  Writes the
- *      best tour (capacity n+K+1), its length and its cost.  Stop rules
- *      come from the ACO_SOLVER_* environment like the reference. ------- */
+/* ---- one-call solver: one instance, one colony, weighted demand and the
+ *      distance mode of choice (the extern "C" form of
+ *      aco_vrp_cuda_with_capacity, src/cuda/aco_cuda.cu:188-454, whose exact
+ *      C++ prototype is in aco_cuda_compat.h).  Writes the best tour (capacity
+ *      n+K+1), its length and its cost in cost units.  Stop rules come from
+ *      the ACO_SOLVER_* environment like the reference. ------------------- */
 int aco_vrp_hip_with_capacity(int n, int K, int capacity, int m, const int32_t *x,
                               const int32_t *y, const int32_t *demand, double alpha,
                               double beta, double rho, double tau0, double Q,
-                              unsigned int seed, int32_t *best_tour, int32_t *best_len,
-                              int64_t *best_cost);
+                              unsigned int seed, int dist_mode, int32_t *best_tour,
+                              int32_t *best_len, int64_t *best_cost);
 
 #ifdef __cplusplus
 }

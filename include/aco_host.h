@@ -95,6 +95,8 @@ bool solution_validate_capacity(const Solution *s, const VrpInstance *inst, char
 int64_t solution_cost_nint(const Solution *s, const VrpInstance *inst);
 /* Real (unrounded, double) route cost, the reference's solution_cost. */
 double solution_cost_real(const Solution *s, const VrpInstance *inst);
+/* Real-mode cost in the solver's units (float distances x 2^23, exact). */
+int64_t solution_cost_real_units(const Solution *s, const VrpInstance *inst);
 
 /* ---- runtime configuration ---------------------------------------------- */
 typedef struct {
@@ -138,6 +140,11 @@ typedef struct {
 void aco_tau_tables_build(AcoTauTables *t, float alpha, float rho, float tau0);
 /* (1/d)^beta for d = 0..dmax, d = 0 uses 1/1e-9 like 1/(0+CUDA_EPS). */
 void aco_eta_table_build(float *eta, int dmax, float beta);
+/* Real mode: eta^beta by heuristic-table index (aco_real_eta_bin of the
+ * float distance), index 0 = distance 0; len = largest index + 1. */
+void aco_eta_table_build_real(float *eta, int len, float beta);
+/* out[i] = the float nearest to sqrt(s[i]), with the kernels' arithmetic. */
+void aco_real_dist_batch(const uint32_t *s, int64_t count, float *out);
 /* Quantize through the threshold table (number of thresholds <= tau). */
 uint8_t aco_tau_quantize(const AcoTauTables *t, float tau);
 /* Host entry to the shared Philox4x32-10 (include/aco_inline.h). */

@@ -96,6 +96,69 @@ static inline ACO_HD int32_t aco_nint_dist(int32_t x1, int32_t y1, int32_t x2, i
   return aco_nint_sqrt_u32(s);
 }
 
+/* ---- real (unrounded) distances ----------------------------------------------
+ * The real distance of an edge is d = the float nearest to sqrt(s), s =
+ * dx^2 + dy^2 <= 2*32767^2 < 2^31 (exactly what (float)sqrt((double)s)
+ * gives with a correctly rounded double square root).  A hardware estimate
+ * from (float)s is within two floats of d.  The midpoint of two neighbouring
+ * floats has at most 25 significant bits, so its square is exact in double
+ * and comparing s with it decides exactly which float is nearest; sqrt(s)
+ * is never such a midpoint (it is an integer below 2^24, hence a float, or
+ * irrational). */
+static inline ACO_HD float aco_bits_float(uint32_t b) {
+  union { uint32_t u; float f; } v;
+  v.u = b;
+  return v.f;
+}
+static inline ACO_HD uint32_t aco_float_bits(float f) {
+  union { uint32_t u; float f; } v;
+  v.f = f;
+  return v.u;
+}
+
+static inline ACO_HD float aco_real_dist_s(uint32_t s) {
+  if (s == 0u) return 0.0f;
+  float d = aco_sqrt_estimate((float)s);
+  const double S = (double)s;
+  for (int it = 0; it < 4; ++it) {
+    const float up = aco_bits_float(aco_float_bits(d) + 1u);
+    const double mu = 0.5 * ((double)d + (double)up);
+    if (S > mu * mu) {
+      d = up;
+      continue;
+    }
+    const float dn = aco_bits_float(aco_float_bits(d) - 1u);
+    const double md = 0.5 * ((double)dn + (double)d);
+    if (S < md * md) {
+      d = dn;
+      continue;
+    }
+    break;
+  }
+  return d;
+}
+
+/* Route-cost units of a real distance: d * 2^23, an exact integer (every
+ * float >= 1 is a multiple of 2^-23), so tour costs are exact int64 sums
+ * whatever the order of the additions.  Cost = units / ACO_REAL_UNITS. */
+#define ACO_REAL_UNITS 8388608.0
+static inline ACO_HD int64_t aco_real_units(float d) {
+  return (int64_t)((double)d * ACO_REAL_UNITS);
+}
+
+/* Heuristic-table index of a real distance: 0 for d = 0, else 1 + the bin of
+ * d among 1024 equal bins of its binade (bins of relative width 2^-10; at
+ * most 15787 for d <= 46341).  The table holds eta at each bin's middle float. */
+#define ACO_REAL_BIN_SHIFT 13
+static inline ACO_HD uint32_t aco_real_eta_bin(float d) {
+  if (!(d >= 1.0f)) return 0u;
+  return 1u + ((aco_float_bits(d) - 0x3F800000u) >> ACO_REAL_BIN_SHIFT);
+}
+static inline ACO_HD float aco_real_bin_centre(uint32_t b) {
+  return aco_bits_float(0x3F800000u + ((b - 1u) << ACO_REAL_BIN_SHIFT) +
+                        (1u << (ACO_REAL_BIN_SHIFT - 1)));
+}
+
 /* Quantize tau through the 255 increasing thresholds thr[b] (the smallest
  * float whose level is >= b+1): level = number of thresholds <= tau. */
 static inline ACO_HD uint32_t aco_quantize_thr(const float *thr, float tau) {

@@ -10,7 +10,10 @@
  * changes the MI355X build makes on purpose:
  *   D1 weighted demand: a candidate is eligible when load + d_j <= Q and the
  *      close rule uses the remaining demand;
- *   D2 TSPLIB nearest-integer distances, integer route costs;
+ *   D2 TSPLIB nearest-integer distances with integer route costs, or (real
+ *      mode) the reference's unrounded distances as floats, with route costs
+ *      kept as exact sums of the float lengths in units of 2^-23 and the
+ *      heuristic (1/d)^beta tabled over bins of relative width 2^-10;
  *   D8 Philox4x32-10 draws keyed by (decision, ant, epoch, colony);
  *   exact roulette: weights are summed in a fixed, documented order (the
  *      order of the wavefront scan) and selection takes the first prefix above
@@ -40,6 +43,7 @@
 
 struct OracleColony {
   int n, K, Q, k, m;
+  int real; /* real-distance mode */
   int32_t *x, *y, *dem;
   int64_t total_demand;
   float alpha, beta, rho, tau0, Qdep, depot_w;
@@ -48,8 +52,8 @@ struct OracleColony {
   float log_min, step;
   uint8_t q_tau0, q_evap;
   float deq[256], thr[255], taupow[256];
-  float *eta_lut; /* (1/d)^beta, d = 0..dmax */
-  int dmax;
+  float *eta_lut; /* (1/d)^beta by distance code 0..dmax */
+  int dmax;       /* largest distance code */
   /* candidates */
   uint32_t *rec;  /* idx | d << 16, (n+1)*k */
   float *ceta;    /* learned heuristic per candidate slot, NULL = (1/d)^beta */
@@ -106,6 +110,38 @@ int32_t oracle_nint_dist(int32_t x1, int32_t y1, int32_t x2, int32_t y2) {
 static uint64_t sq_dist(const OracleColony *o, int a, int b) {
   int64_t dx = o->x[a] - o->x[b], dy = o->y[a] - o->y[b];
   return (uint64_t)(dx * dx + dy * dy);
+}
+
+/* ---- real mode (instance_parser.c:17-19, aco_cuda_kernels.cu:53-57) ------ */
+
+/* The reference's unrounded distance as a float: the double root (glibc's
+ * sqrt is correctly rounded) rounded once to float. */
+static float real_dist_s(uint64_t s) { return (float)sqrt((double)s); }
+
+/* Heuristic-table code of a float distance: 0 for d = 0, else 1 + its bin
+ * among 1024 bins per binade of the float's bit pattern. */
+static uint32_t real_bin(float d) {
+  if (!(d >= 1.0f)) return 0;
+  uint32_t bits;
+  memcpy(&bits, &d, 4);
+  return 1u + ((bits - 0x3F800000u) >> 13);
+}
+
+/* Distance code of an edge: nint distance, or the bin of the float distance. */
+static uint32_t dist_code(const OracleColony *o, int a, int b) {
+  if (o->real) return real_bin(real_dist_s(sq_dist(o, a, b)));
+  return (uint32_t)oracle_nint_dist(o->x[a], o->y[a], o->x[b], o->y[b]);
+}
+
+/* Cost of an edge: nint distance, or the float distance in units of 2^-23. */
+static int64_t edge_cost(const OracleColony *o, int a, int b) {
+  if (o->real) return (int64_t)((double)real_dist_s(sq_dist(o, a, b)) * 8388608.0);
+  return oracle_nint_dist(o->x[a], o->y[a], o->x[b], o->y[b]);
+}
+
+/* Tour length the deposits divide by. */
+static float cost_len(const OracleColony *o, int64_t c) {
+  return o->real ? (float)c * (1.0f / 8388608.0f) : (float)c;
 }
 
 /* Inclusive prefix sums of up to 64 lane values in the order of the
@@ -197,16 +233,13 @@ static void build_candidates(OracleColony *o, int knn_key) {
       uint32_t idx = 0, d = 0;
       if (t < cnt) {
         idx = (uint32_t)(items[t].key & 0xFFFFFFu);
-        d = (uint32_t)oracle_nint_dist(o->x[i], o->y[i], o->x[idx], o->y[idx]);
+        d = dist_code(o, i, (int)idx);
       }
       o->rec[(size_t)i * k + t] = idx | (d << 16);
     }
   }
   free(items);
-  for (int i = 0; i <= n; ++i) {
-    int d0 = oracle_nint_dist(o->x[i], o->y[i], o->x[0], o->y[0]);
-    o->dscore[i] = o->depot_w * o->eta_lut[d0];
-  }
+  for (int i = 0; i <= n; ++i) o->dscore[i] = o->depot_w * o->eta_lut[dist_code(o, i, 0)];
 }
 
 /* ---- lifecycle ----------------------------------------------------------- */
@@ -214,9 +247,10 @@ static void build_candidates(OracleColony *o, int knn_key) {
 OracleColony *oracle_create(int n, int K, int Q, const int32_t *x, const int32_t *y,
                             const int32_t *demand, int cand_k, int ants, float alpha, float beta,
                             float rho, float tau0, float Qdep, float depot_w, int knn_key,
-                            uint32_t seed, uint32_t colony) {
+                            uint32_t seed, uint32_t colony, int dist_mode) {
   if (n <= 0 || K <= 0 || Q <= 0 || ants <= 0 || cand_k <= 0) return NULL;
   OracleColony *o = (OracleColony *)calloc(1, sizeof(*o));
+  o->real = dist_mode == 1;
   o->n = n; o->K = K; o->Q = Q; o->m = ants;
   o->k = cand_k > n ? n : cand_k; /* aco_cuda.cu:198-199 clamps k to n */
   if (o->k > 64) o->k = 64;
@@ -237,11 +271,28 @@ OracleColony *oracle_create(int n, int K, int Q, const int32_t *x, const int32_t
     if (y[i] < ymin) ymin = y[i];
     if (y[i] > ymax) ymax = y[i];
   }
-  o->dmax = oracle_nint_dist(xmin, ymin, xmax, ymax);
+  /* the bounding box's diagonal has the largest distance code */
+  if (o->real) {
+    const uint64_t dx = (uint64_t)(xmax - xmin), dy = (uint64_t)(ymax - ymin);
+    o->dmax = (int)real_bin(real_dist_s(dx * dx + dy * dy));
+  } else {
+    o->dmax = oracle_nint_dist(xmin, ymin, xmax, ymax);
+  }
   build_tables(o);
   o->eta_lut = (float *)malloc(sizeof(float) * (size_t)(o->dmax + 1));
   for (int d = 0; d <= o->dmax; ++d) {
-    double base = (d == 0) ? (double)1e-9f : (double)d;
+    double base;
+    if (d == 0) {
+      base = (double)1e-9f;
+    } else if (o->real) {
+      /* the middle float of bin d: bits 1.0f + (d-1)*2^13 + 2^12 */
+      const uint32_t bits = 0x3F800000u + ((uint32_t)(d - 1) << 13) + (1u << 12);
+      float c;
+      memcpy(&c, &bits, 4);
+      base = (double)c;
+    } else {
+      base = (double)d;
+    }
     o->eta_lut[d] = (float)pow(base, -(double)beta);
   }
   o->rec = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1) * o->k);
@@ -359,10 +410,8 @@ static int fallback_select(const OracleColony *o, const uint16_t *state, int cur
       for (int i = 0; i < FB_V; ++i) {
         int j = c * FB_CHUNK + l * FB_V + i;
         float wt = 0.0f;
-        if (j <= n && (int)state[j] <= room) {
-          int d = oracle_nint_dist(o->x[cur], o->y[cur], o->x[j], o->y[j]);
-          wt = o->taupow[trow[j]] * o->eta_lut[d];
-        }
+        if (j <= n && (int)state[j] <= room)
+          wt = o->taupow[trow[j]] * o->eta_lut[dist_code(o, cur, j)];
         s = s + wt;
       }
       lanes[l] = s;
@@ -391,10 +440,8 @@ static int fallback_select(const OracleColony *o, const uint16_t *state, int cur
     for (int i = 0; i < FB_V; ++i) {
       int j = cs * FB_CHUNK + l * FB_V + i;
       float wt = 0.0f;
-      if (j <= n && (int)state[j] <= room) {
-        int d = oracle_nint_dist(o->x[cur], o->y[cur], o->x[j], o->y[j]);
-        wt = o->taupow[trow[j]] * o->eta_lut[d];
-      }
+      if (j <= n && (int)state[j] <= room)
+        wt = o->taupow[trow[j]] * o->eta_lut[dist_code(o, cur, j)];
       w[l][i] = wt;
       s = s + wt;
     }
@@ -501,8 +548,7 @@ int oracle_construct_ant(const OracleColony *o, int ant, int epoch, int32_t *tou
   }
   int len = pos + 1;
   int64_t cost = 0;
-  for (int p = 0; p + 1 < len; ++p)
-    cost += oracle_nint_dist(o->x[tour[p]], o->y[tour[p]], o->x[tour[p + 1]], o->y[tour[p + 1]]);
+  for (int p = 0; p + 1 < len; ++p) cost += edge_cost(o, tour[p], tour[p + 1]);
   *cost_out = cost;
   *unvisited_out = rem_cnt;
   if (fallbacks_out) *fallbacks_out = fallbacks;
@@ -591,8 +637,8 @@ void oracle_select_best(OracleColony *o) {
       st->iter_since_best++;
     }
     const float qeff = o->Qdep * 100.0f;
-    st->dep_iter = (0.3f * qeff) / (float)c;
-    st->dep_best = (0.7f * qeff) / (float)st->global_best_cost;
+    st->dep_iter = (0.3f * qeff) / cost_len(o, c);
+    st->dep_best = (0.7f * qeff) / cost_len(o, st->global_best_cost);
   } else {
     st->iter_best_cost = INT64_MAX;
     st->iter_since_best++;
@@ -676,7 +722,6 @@ int oracle_get_best(const OracleColony *o, int32_t *tour, int32_t *len, int64_t 
 
 int64_t oracle_route_cost(const OracleColony *o, const int32_t *tour, int len) {
   int64_t c = 0;
-  for (int p = 0; p + 1 < len; ++p)
-    c += oracle_nint_dist(o->x[tour[p]], o->y[tour[p]], o->x[tour[p + 1]], o->y[tour[p + 1]]);
+  for (int p = 0; p + 1 < len; ++p) c += edge_cost(o, tour[p], tour[p + 1]);
   return c;
 }

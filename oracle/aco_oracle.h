@@ -20,10 +20,12 @@ typedef struct {
   float dep_iter, dep_best;
 } OracleState;
 
+/* dist_mode: 0 = TSPLIB nint distances and integer costs, 1 = real
+ * distances (floats), costs in units of 2^-23. */
 OracleColony *oracle_create(int n, int K, int Q, const int32_t *x, const int32_t *y,
                             const int32_t *demand, int cand_k, int ants, float alpha, float beta,
                             float rho, float tau0, float Qdep, float depot_w, int knn_key,
-                            uint32_t seed, uint32_t colony);
+                            uint32_t seed, uint32_t colony, int dist_mode);
 void oracle_destroy(OracleColony *o);
 int oracle_cand_k(const OracleColony *o);
 int oracle_tour_stride(const OracleColony *o);

@@ -32,7 +32,7 @@ class OracleState(C.Structure):
 _SIG = {
     "oracle_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, C.c_int,
                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
-                                   C.c_float, C.c_int, C.c_uint32, C.c_uint32]),
+                                   C.c_float, C.c_int, C.c_uint32, C.c_uint32, C.c_int]),
     "oracle_destroy": (None, [C.c_void_p]),
     "oracle_cand_k": (C.c_int, [C.c_void_p]),
     "oracle_tour_stride": (C.c_int, [C.c_void_p]),
@@ -89,7 +89,7 @@ class Oracle:
     """One colony of the CPU restatement (same parameters as the GPU context)."""
 
     def __init__(self, inst, cand_k=20, ants=256, alpha=1.0, beta=2.0, rho=0.1, tau0=1.0, Q=1.0,
-                 depot_w=2.0, knn_key=0, seed=1234, colony=0):
+                 depot_w=2.0, knn_key=0, seed=1234, colony=0, dist_mode=0):
         self.lib = load()
         self.inst = inst
         self.x = np.ascontiguousarray(inst.x, np.int32)
@@ -98,7 +98,7 @@ class Oracle:
         self.h = self.lib.oracle_create(inst.n, inst.K, inst.capacity, _p(self.x, C.c_int32),
                                         _p(self.y, C.c_int32), _p(self.d, C.c_int32), cand_k,
                                         ants, alpha, beta, rho, tau0, Q, depot_w, knn_key, seed,
-                                        colony)
+                                        colony, dist_mode)
         if not self.h:
             raise ValueError("oracle_create rejected its arguments")
         self.n, self.m = inst.n, ants

@@ -176,3 +176,57 @@ def test_quality_fixture_instances_regenerate():
             h.update(np.ascontiguousarray(a, np.int32).tobytes())
         assert h.hexdigest() == r["instance_sha1"], r["instance_seed"]
         assert r["epochs"] >= 50 and r["best_cost"] > 0
+
+
+def test_drop_in_entry_point_exported_and_refusals():
+    """aco_vrp_cuda_with_capacity is exported with the reference's C++
+    prototype (src/cuda/main_vrp.cu:29-35, mangled name below) and refuses,
+    before any device work, what the reference's checks refuse
+    (aco_cuda.cu:193-196) plus non-integral or out-of-range coordinates."""
+    lib = N.load()
+    f = lib["_Z26aco_vrp_cuda_with_capacityiiiiPfS_dddddjP8SolutionPd"]
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
+                  C.POINTER(C.c_float), C.c_double, C.c_double, C.c_double, C.c_double,
+                  C.c_double, C.c_uint, C.c_void_p, C.POINTER(C.c_double)]
+    lib.solution_create.restype = C.c_void_p
+    lib.solution_create.argtypes = [C.c_int, C.c_int]
+    lib.solution_free.argtypes = [C.c_void_p]
+    n, K = 8, 2
+    sol = lib.solution_create(K, n)
+    cost = C.c_double(0.0)
+    fp = C.POINTER(C.c_float)
+
+    def call(x, y, n_=n, K_=K, cap=5, s=sol):
+        xa = np.ascontiguousarray(x, np.float32)
+        ya = np.ascontiguousarray(y, np.float32)
+        return f(n_, K_, cap, 0, xa.ctypes.data_as(fp), ya.ctypes.data_as(fp), 1.0, 2.0, 0.5,
+                 1.0, 1.0, 1234, s, C.byref(cost))
+
+    x = [50, 10, 20, 10, 20, 80, 90, 80, 90]
+    y = [50, 10, 10, 20, 20, 80, 80, 90, 90]
+    assert call(x, y, n_=0) == 1
+    assert call(x, y, K_=0) == 1
+    assert call(x, y, s=None) == 1
+    assert call(x, y, cap=0) == 1
+    assert call([50, 10.5] + x[2:], y) == 1
+    assert call(x, [50, -1] + y[2:]) == 1
+    assert call([50, 40000] + x[2:], y) == 1
+    lib.solution_free(sol)
+
+
+def test_reference_cli_links_against_library():
+    """The reference's own CUDA CLI source (src/cuda/main_vrp.cu, compiled as
+    C++ with its common sources by oracle/ref.mk in the build container)
+    links against libaco_hip.so: its only solver symbol resolves there."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "aco_vrp_cuda_linked")
+    if not os.path.exists(exe):
+        pytest.skip("reference sources not present (oracle/ref.mk builds the link check)")
+    und = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True,
+                         text=True).stdout
+    assert "_Z26aco_vrp_cuda_with_capacityiiiiPfS_dddddjP8SolutionPd" in und
+    exported = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True,
+                              text=True).stdout
+    assert "_Z26aco_vrp_cuda_with_capacityiiiiPfS_dddddjP8SolutionPd" in exported
+    ldd = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "libaco_hip.so" in ldd and "not found" not in ldd.split("libaco_hip.so")[1].split("\n")[0]

@@ -34,8 +34,8 @@ def test_oracle_clean_under_sanitizers(tmp_path):
     assert r.returncode == 0, r.stderr[-4000:]
     assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr
     lines = [l for l in r.stdout.splitlines() if l.startswith("case ")]
-    assert len(lines) == 6
-    assert "feasible_ants=0" in lines[4]  # the capacity-short case
+    assert len(lines) == 12  # six cases in each distance mode
+    assert "feasible_ants=0" in lines[4] and "feasible_ants=0" in lines[10]  # capacity-short
 
 
 def test_vrp_io_clean_under_sanitizers(tmp_path):

@@ -23,6 +23,10 @@ ACO_HIP_ERR_BACKEND = 4
 KNN_EXACT = 0
 KNN_REF_F32 = 1
 
+DIST_NINT = 0
+DIST_REAL = 1
+REAL_COST_UNIT = 1.0 / 8388608.0  # ACO_HIP_REAL_COST_UNIT: real length per cost unit
+
 i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
 u8p = C.POINTER(C.c_uint8)
@@ -47,7 +51,7 @@ class AcoHipParams(C.Structure):
                 ("alpha", C.c_float), ("beta", C.c_float), ("rho", C.c_float),
                 ("tau0", C.c_float), ("Q", C.c_float), ("depot_close_weight", C.c_float),
                 ("knn_key", C.c_int), ("ants_per_group", C.c_int), ("stage_lds", C.c_int),
-                ("fallback_matrix", C.c_int)]
+                ("fallback_matrix", C.c_int), ("dist_mode", C.c_int)]
 
 
 class AcoHipColonyState(C.Structure):
@@ -141,7 +145,7 @@ HIP_SYMBOLS = {
     "aco_hip_get_timing": (C.c_int, [ctx_p, C.POINTER(AcoHipTiming)]),
     "aco_vrp_hip_with_capacity": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, i32p, i32p, i32p,
                                             C.c_double, C.c_double, C.c_double, C.c_double,
-                                            C.c_double, C.c_uint, i32p, i32p, i64p]),
+                                            C.c_double, C.c_uint, C.c_int, i32p, i32p, i64p]),
     "aco_hip_num_buckets": (C.c_int, [ctx_p]),
     "aco_hip_get_bucket": (C.c_int, [ctx_p, C.c_int, C.POINTER(AcoHipBucketInfo)]),
     "aco_hip_set_bucket_shape": (C.c_int, [ctx_p, C.c_int, C.c_int, C.c_int]),
@@ -176,6 +180,9 @@ HOST_SYMBOLS = {
     "vrp_real_dist": (C.c_double, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "aco_tau_tables_build": (None, [C.POINTER(AcoTauTables), C.c_float, C.c_float, C.c_float]),
     "aco_eta_table_build": (None, [f32p, C.c_int, C.c_float]),
+    "aco_eta_table_build_real": (None, [f32p, C.c_int, C.c_float]),
+    "aco_real_dist_batch": (None, [C.POINTER(C.c_uint32), C.c_int64, f32p]),
+    "solution_cost_real_units": (C.c_int64, [C.c_void_p, C.c_void_p]),
     "aco_tau_quantize": (C.c_uint8, [C.POINTER(AcoTauTables), C.c_float]),
     "aco_parse_min_rel_improvement_percent": (C.c_double, [C.c_char_p, C.c_double]),
     "aco_is_significant_improvement": (C.c_int, [C.c_double, C.c_double, C.c_double]),

@@ -277,6 +277,8 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if (!(p->tau0 > 0.0f)) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tau0 must be > 0");
   if (p->knn_key != ACO_HIP_KNN_EXACT && p->knn_key != ACO_HIP_KNN_REF_F32)
     return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "unknown knn_key");
+  if (p->dist_mode != ACO_HIP_DIST_NINT && p->dist_mode != ACO_HIP_DIST_REAL)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "unknown dist_mode");
   HIPC(c, hipSetDevice(c->device));
   HIPC(c, hipStreamSynchronize(c->stream));
   free_pools(c);
@@ -308,7 +310,14 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       xmin = std::min(xmin, a.x[j]); xmax = std::max(xmax, a.x[j]);
       ymin = std::min(ymin, a.y[j]); ymax = std::max(ymax, a.y[j]);
     }
-    dmax = std::max(dmax, (int)aco_nint_dist(xmin, ymin, xmax, ymax));
+    /* largest distance code of the instance: that of its bounding box's
+     * diagonal (codes grow with the distance) */
+    if (p->dist_mode == ACO_HIP_DIST_REAL) {
+      const uint32_t dx = (uint32_t)(xmax - xmin), dy = (uint32_t)(ymax - ymin);
+      dmax = std::max(dmax, (int)aco_real_eta_bin(aco_real_dist_s(dx * dx + dy * dy)));
+    } else {
+      dmax = std::max(dmax, (int)aco_nint_dist(xmin, ymin, xmax, ymax));
+    }
     InstDesc &d = c->inst[i];
     d.scale = (float)std::max(1, std::max(xmax - xmin, ymax - ymin));
     d.n = a.n;
@@ -340,7 +349,8 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   aco_tau_tables_build(&c->tables, p->alpha, p->rho, p->tau0);
   c->eta_len = dmax + 1;
   c->eta_lut.assign((size_t)c->eta_len, 0.0f);
-  aco_eta_table_build(c->eta_lut.data(), dmax, p->beta);
+  if (p->dist_mode == ACO_HIP_DIST_REAL) aco_eta_table_build_real(c->eta_lut.data(), c->eta_len, p->beta);
+  else aco_eta_table_build(c->eta_lut.data(), dmax, p->beta);
 
   /* colonies */
   c->col.assign(c->num_col, ColonyDesc{});
@@ -606,6 +616,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.eta = c->d_eta;
   P.eta_len = c->eta_len;
   P.eta_lds = c->eta_lds;
+  P.dist_real = p->dist_mode == ACO_HIP_DIST_REAL ? 1 : 0;
   P.q_evap = c->tables.q_evap;
   P.qdep = p->Q;
   P.fixed_epochs = 0;
@@ -1056,8 +1067,14 @@ static int capture_chunk(AcoHipContext *c, int chunk) {
   return ACO_HIP_OK;
 }
 
+/* A colony cost in the units of the printed cost: the nint length, or the
+ * real length in real mode. */
+static double printed_cost(const AcoHipContext *c, long long cost) {
+  return c->P.dist_real ? (double)cost * ACO_HIP_REAL_COST_UNIT : (double)cost;
+}
+
 static void print_progress(int epoch, double elapsed, double max_rt, int since, int max_stag,
-                           long long best) {
+                           long long best, double unit) {
   char tr[32], er[32], bc[64];
   if (max_rt > 0.0) snprintf(tr, sizeof(tr), "%.1fs", std::max(0.0, max_rt - elapsed));
   else snprintf(tr, sizeof(tr), "senza limite");
@@ -1068,7 +1085,7 @@ static void print_progress(int epoch, double elapsed, double max_rt, int since, 
   } else {
     snprintf(er, sizeof(er), "senza limite");
   }
-  if (best != INT64_MAX) snprintf(bc, sizeof(bc), "%.3f", (double)best);
+  if (best != INT64_MAX) snprintf(bc, sizeof(bc), "%.3f", (double)best * unit);
   else snprintf(bc, sizeof(bc), "non disponibile");
   printf("[progress][hip] epoca=%d | epoche_rimanenti_prima_stop=%s | tempo_trascorso=%.1fs | "
          "tempo_rimanente=%s | best_cost=%s\n",
@@ -1142,13 +1159,14 @@ extern "C" int aco_hip_run(AcoHipContext *c, const AcoHipRunConfig *cfg, AcoHipR
         const int reached_stag = h.halted;
         if (e == 1 || e % 10 == 0 || h.improved || reached_time || reached_stag)
           print_progress(e, elapsed, cfg->timeout_seconds, h.iter_since_best,
-                         cfg->stagnation_epochs, h.global_best);
+                         cfg->stagnation_epochs, h.global_best,
+                         c->P.dist_real ? ACO_HIP_REAL_COST_UNIT : 1.0);
       }
       printed_upto = st[0].epoch;
     }
     if (next_beat > 0 && now >= next_beat) {
       fprintf(stderr, "[hip] elapsed %.1fs, iter %d, best %.3f\n", elapsed, st[0].epoch,
-              st[0].global_best_cost == INT64_MAX ? 0.0 : (double)st[0].global_best_cost);
+              st[0].global_best_cost == INT64_MAX ? 0.0 : printed_cost(c, st[0].global_best_cost));
       next_beat = now + cfg->progress_interval_seconds;
     }
     bool all_halted = true;
@@ -1493,8 +1511,8 @@ extern "C" int aco_hip_get_timing(AcoHipContext *c, AcoHipTiming *t) {
 extern "C" int aco_vrp_hip_with_capacity(int n, int K, int capacity, int m, const int32_t *x,
                                          const int32_t *y, const int32_t *demand, double alpha,
                                          double beta, double rho, double tau0, double Q,
-                                         unsigned int seed, int32_t *best_tour, int32_t *best_len,
-                                         int64_t *best_cost) {
+                                         unsigned int seed, int dist_mode, int32_t *best_tour,
+                                         int32_t *best_len, int64_t *best_cost) {
   if (n <= 0 || K <= 0 || !x || !y || !demand || !best_tour || !best_len || !best_cost)
     return ACO_HIP_ERR_INVALID_INPUT;
   AcoRuntimeConfig rt;
@@ -1518,6 +1536,7 @@ extern "C" int aco_vrp_hip_with_capacity(int n, int K, int capacity, int m, cons
   p.Q = (float)Q;
   p.depot_close_weight = 2.0f;
   p.knn_key = env_int("ACO_HIP_KNN_KEY", ACO_HIP_KNN_EXACT);
+  p.dist_mode = dist_mode;
   p.ants_per_group = env_int("ACO_HIP_ANTS_PER_GROUP", 0);
   p.stage_lds = env_int("ACO_HIP_STAGE_LDS", -1);
   if ((rc = aco_hip_load(c, &in, 1, &p)) || (rc = aco_hip_build_candidates(c)) ||

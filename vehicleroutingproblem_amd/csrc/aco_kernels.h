@@ -26,7 +26,8 @@
  *   tours   int32  [colonies][ants][stride] step-interleaved routes
  *   len/unvis int32, cost int64 [colonies*ants]
  *   best    int32  [colonies][stride]   global-best tour per colony
- *   tables  deq[256], thr[255], taupow[256], eta[dmax+1]
+ *   tables  deq[256], thr[255], taupow[256], eta[code] (code = nint distance,
+ *           or in real mode the bin of the float distance)
  */
 #ifndef ACO_KERNELS_H
 #define ACO_KERNELS_H
@@ -36,6 +37,7 @@
 #include <stdint.h>
 
 #include "aco_hip.h"
+#include "aco_inline.h"
 
 struct InstDesc {
   int n, K, k, Q;
@@ -93,6 +95,7 @@ struct DevCtx {
   const float *eta;
   int eta_len;
   int eta_lds;         /* the eta table is staged in LDS (else read from global) */
+  int dist_real;       /* ACO_HIP_DIST_REAL: float distances, costs in 2^-23 units */
   uint32_t q_evap;
   float qdep;
   int fixed_epochs;
@@ -125,8 +128,38 @@ struct HistEntry {
 
 static inline __host__ __device__ size_t round16(size_t x) { return (x + 15) & ~(size_t)15; }
 
+/* Squared length of an edge (coordinates in [0, 32767]: exact in 32 bits). */
+static inline __device__ uint32_t edge_sq(int2 a, int2 b) {
+  return aco_sq15(a.x - b.x) + aco_sq15(a.y - b.y);
+}
+
+/* Distance code of an edge, the index of its (1/d)^beta in the eta table:
+ * the nint distance, or in real mode the bin of the float distance. */
+template <bool REAL>
+static inline __device__ uint32_t edge_code_t(int2 a, int2 b) {
+  const uint32_t s = edge_sq(a, b);
+  return REAL ? aco_real_eta_bin(aco_real_dist_s(s)) : (uint32_t)aco_nint_sqrt_u32(s);
+}
+static inline __device__ uint32_t edge_code(const DevCtx &P, int2 a, int2 b) {
+  return P.dist_real ? edge_code_t<true>(a, b) : edge_code_t<false>(a, b);
+}
+
+/* Cost of an edge in cost units: the nint distance, or in real mode the
+ * float distance times 2^23 (exact). */
+static inline __device__ long long edge_units(const DevCtx &P, int2 a, int2 b) {
+  const uint32_t s = edge_sq(a, b);
+  return P.dist_real ? (long long)aco_real_units(aco_real_dist_s(s))
+                     : (long long)aco_nint_sqrt_u32(s);
+}
+
+/* Tour length L of a cost in cost units, as the float the deposits divide
+ * by: (float)cost, times 2^-23 in real mode (exact scaling). */
+static inline __host__ __device__ float cost_len(int dist_real, long long cost) {
+  return dist_real ? (float)cost * (float)ACO_HIP_REAL_COST_UNIT : (float)cost;
+}
+
 /* Heuristic factor of candidate slot e (0 <= e < (n+1)k) of instance I: the
- * learned prior when set, else (1/d)^beta of the slot's nint distance. */
+ * learned prior when set, else (1/d)^beta of the slot's distance code. */
 static inline __device__ float slot_eta(const DevCtx &P, const InstDesc &I, size_t e,
                                         uint32_t rec) {
   return I.learned ? P.ceta[I.cand_off + e] : P.eta[rec >> 16];

@@ -88,7 +88,8 @@ __device__ __forceinline__ int last_lane(uint64_t m) { return 63 - __builtin_clz
  * primary dx^2+dy^2 (exact) or the float bits of (float)sqrt((double)s)
  * (reference OpenMP order).  Follows kernel_build_candidate_lists
  * (aco_cuda_kernels.cu:164-214): customers only, self excluded, ties to the
- * lower index, slots past n-1 padded with index 0. */
+ * lower index, slots past n-1 padded with index 0.  Each record carries the
+ * slot's distance code (the eta-table index) in its upper 16 bits. */
 __global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_key,
                                               float depot_w) {
   const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -124,8 +125,7 @@ __global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_k
       uint32_t r = 0;
       if (best != ~0ull) {
         const int j = (int)(best & 0xFFFFFFu);
-        const int2 pj = xy[j];
-        const uint32_t d = (uint32_t)aco_nint_dist(pi.x, pi.y, pj.x, pj.y);
+        const uint32_t d = edge_code(P, pi, xy[j]);
         r = (uint32_t)j | (d << 16);
       }
       rec[t] = r;
@@ -134,9 +134,7 @@ __global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_k
     last = best;
   }
   if (lane == 0) {
-    const int2 p0 = xy[0];
-    const int d0 = aco_nint_dist(pi.x, pi.y, p0.x, p0.y);
-    const float ds = depot_w * P.eta[d0];
+    const float ds = depot_w * P.eta[edge_code(P, pi, xy[0])];
     P.dscore[I.node_off + i] = ds == ds ? ds : 0.0f; /* NaN scores never close a route */
   }
 }
@@ -234,7 +232,7 @@ __device__ __forceinline__ float fb_lane_fs(float (&w)[FB_V], int b0, int n1, in
 __device__ __forceinline__ float fb_lane_tables(float (&w)[FB_V], int b0, int n1, int room,
                                                 int2 pc, const int2 *xy, const float *taupow,
                                                 const float *eta, const uint16_t *state,
-                                                const uint8_t *__restrict__ trow) {
+                                                const uint8_t *__restrict__ trow, bool real) {
   uint32_t st[FB_V], tq[FB_V];
   int2 pj[FB_V];
 #pragma unroll
@@ -248,7 +246,7 @@ __device__ __forceinline__ float fb_lane_tables(float (&w)[FB_V], int b0, int n1
   float s = 0.0f;
 #pragma unroll
   for (int i = 0; i < FB_V; ++i) {
-    const int d = aco_nint_dist(pc.x, pc.y, pj[i].x, pj[i].y);
+    const uint32_t d = real ? edge_code_t<true>(pc, pj[i]) : edge_code_t<false>(pc, pj[i]);
     const float v = taupow[tq[i]] * eta[d];
     w[i] = (b0 + i < n1 && (int)st[i] <= room) ? v : 0.0f;
     s = s + w[i];
@@ -266,7 +264,7 @@ __device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow,
                                               const float *taupow, const float *eta,
                                               const uint16_t *state,
                                               const uint8_t *__restrict__ trow, int n1, int cur,
-                                              int room, float u, int lane) {
+                                              int room, float u, int lane, bool real) {
   const int nch = (n1 + FB_CHUNK - 1) / FB_CHUNK;
   const int2 pc = xy[cur];
   float w[FB_V];
@@ -274,7 +272,7 @@ __device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow,
   for (int c = 0; c < nch; ++c) {
     const int b0 = c * FB_CHUNK + lane * FB_V;
     s = frow ? fb_lane_fs(w, b0, n1, room, frow, state)
-             : fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow);
+             : fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow, real);
     Pl = wave_scan_rows(s, lane, 4);
     const float Cc = f_readlane(Pl, 63);
     R = R + Cc;
@@ -296,7 +294,7 @@ __device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow,
   if (cs != nch - 1) {
     const int b0 = cs * FB_CHUNK + lane * FB_V;
     s = frow ? fb_lane_fs(w, b0, n1, room, frow, state)
-             : fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow);
+             : fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow, real);
     Pl = wave_scan_rows(s, lane, 4);
   }
   const uint64_t lm = ballot64(s > 0.0f && Pl > thr_c);
@@ -518,7 +516,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
         const unsigned long long f0 = STATS ? clock64() : 0ull;
         const FbOut fb = fallback_select(fsm ? fsm + (size_t)cur * Sn : nullptr, c_xy, c_taupow,
                                          c_eta, s_state, tau + (size_t)cur * Sn, n1, cur, room,
-                                         u0, lane);
+                                         u0, lane, P.dist_real != 0);
         move = fb.move;
         mdem = fb.mdem;
         T = fb.T;
@@ -573,15 +571,12 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  /* write the tour out and sum its nint edge lengths (int64, exact) */
+  /* write the tour out and sum its edge costs (int64 cost units, exact) */
   long long cost = 0;
   for (int p = lane; p <= pos; p += 64) {
     const int v = s_tour[p];
     tour_out[p] = v;
-    if (p >= 1) {
-      const int2 a = c_xy[s_tour[p - 1]], b = c_xy[v];
-      cost += aco_nint_dist(a.x, a.y, b.x, b.y);
-    }
+    if (p >= 1) cost += edge_units(P, c_xy[s_tour[p - 1]], c_xy[v]);
   }
   cost = wave_sum_i64(cost);
   if (lane == 0) {
@@ -668,7 +663,7 @@ __device__ __forceinline__ void fs_item(const DevCtx &P, const ColonyDesc &D, co
     const int px = (i & 1) ? q[i >> 1].z : q[i >> 1].x;
     const int py = (i & 1) ? q[i >> 1].w : q[i >> 1].y;
     const uint32_t b = (tw[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-    const int d = aco_nint_dist(pu.x, pu.y, px, py);
+    const uint32_t d = edge_code(P, pu, make_int2(px, py));
     o[i] = v0 + i < n1 ? taupow[b] * eta[v0 + i < n1 ? d : 0] : 0.0f;
   }
   float4 *f4 = reinterpret_cast<float4 *>(fs + row);
@@ -684,6 +679,7 @@ __device__ __forceinline__ void fs_item(const DevCtx &P, const ColonyDesc &D, co
  * instruction).  The quad's pheromone word t is already loaded; once the
  * coordinate loads are issued, the word at `next` is loaded into t_next for
  * the thread's next quad, so its latency overlaps this quad's arithmetic. */
+template <bool REAL>
 __device__ __forceinline__ void fs_quad_evap(const DevCtx &P, const ColonyDesc &D,
                                              const InstDesc &I, const float *taupow,
                                              const float *eta, uint32_t it, uint32_t r,
@@ -716,7 +712,7 @@ __device__ __forceinline__ void fs_quad_evap(const DevCtx &P, const ColonyDesc &
     /* past the last node the coordinates are the pool's padding (in
      * bounds); the table reads are unconditional, at index 0 there */
     const bool live = v0 + i < n1;
-    const int d = aco_nint_dist(pu.x, pu.y, px[i], py[i]);
+    const uint32_t d = edge_code_t<REAL>(pu, make_int2(px[i], py[i]));
     const uint32_t b = (t >> (8 * i)) & 0xFFu;
     const float w = taupow[b] * eta[live ? d : 0];
     o[i] = live ? w : 0.0f;
@@ -886,9 +882,15 @@ __global__ void __launch_bounds__(512) k_evaporate_flat(DevCtx P, const long lon
        * that the wait counts are the same on every path */
       const uint32_t *nx = tw + (pf ? qe + blockDim.x : qe);
       const uint32_t it = (uint32_t)(g - lo), r = (uint32_t)(qq & 3);
-      /* two copies, so that the LDS table is read as LDS */
-      if (eta_lds) fs_quad_evap(P, D, I, s_tp, s_eta, it, r, t, d, nx, tn);
-      else fs_quad_evap(P, D, I, s_tp, P.eta, it, r, t, d, nx, tn);
+      /* copies per table location (so that the LDS table is read as LDS)
+       * and per distance mode */
+      if (P.dist_real) {
+        if (eta_lds) fs_quad_evap<true>(P, D, I, s_tp, s_eta, it, r, t, d, nx, tn);
+        else fs_quad_evap<true>(P, D, I, s_tp, P.eta, it, r, t, d, nx, tn);
+      } else {
+        if (eta_lds) fs_quad_evap<false>(P, D, I, s_tp, s_eta, it, r, t, d, nx, tn);
+        else fs_quad_evap<false>(P, D, I, s_tp, P.eta, it, r, t, d, nx, tn);
+      }
       qn = pf ? q2 : -1;
     }
     gs = g0 > ti ? g0 : ti;
@@ -944,45 +946,79 @@ struct UpdTables {
   const float *eta;    /* eta table (global memory, L2-resident) */
 };
 
+struct BestAnt {
+  long long cost; /* INT64_MAX when no ant is feasible */
+  int ant;        /* -1 when no ant is feasible */
+  int feasible;   /* feasible ants */
+};
+
+/* Iteration best of a colony (select_iter_best_host, aco_cuda.cu:104-126):
+ * the lowest cost among feasible ants, ties to the lower ant, as two
+ * block-wide minimum reductions (the cost, then the lowest ant at that
+ * cost), so costs of any size compare exactly.  `skip` (uniform) reports no
+ * feasible ant.  Every thread of the block gets the result. */
+__device__ BestAnt block_best_ant(const DevCtx &P, const ColonyDesc &D, bool skip) {
+  __shared__ unsigned long long s_red[16];
+  __shared__ int s_cnt[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  unsigned long long c = ~0ull;
+  int cnt = 0;
+  if (!skip)
+    for (int a = threadIdx.x; a < D.m; a += blockDim.x)
+      if (P.unvis[D.ant_off + a] == 0) {
+        const unsigned long long ca = (unsigned long long)P.cost[D.ant_off + a];
+        c = ca < c ? ca : c;
+        ++cnt;
+      }
+  c = wave_min_u64(c);
+  cnt = (int)wave_sum_i64(cnt);
+  if (lane == 0) {
+    s_red[wave] = c;
+    s_cnt[wave] = cnt;
+  }
+  __syncthreads();
+  unsigned long long best = s_red[0];
+  int feas = s_cnt[0];
+  for (int w = 1; w < nw; ++w) {
+    best = s_red[w] < best ? s_red[w] : best;
+    feas += s_cnt[w];
+  }
+  __syncthreads();
+  unsigned long long am = ~0ull;
+  if (best != ~0ull)
+    for (int a = threadIdx.x; a < D.m; a += blockDim.x)
+      if (P.unvis[D.ant_off + a] == 0 && (unsigned long long)P.cost[D.ant_off + a] == best) {
+        am = (unsigned long long)a;
+        break; /* a thread's ants ascend: its first match is its lowest */
+      }
+  am = wave_min_u64(am);
+  if (lane == 0) s_red[wave] = am;
+  __syncthreads();
+  unsigned long long ant = s_red[0];
+  for (int w = 1; w < nw; ++w) ant = s_red[w] < ant ? s_red[w] : ant;
+  __syncthreads();
+  BestAnt r;
+  r.cost = best == ~0ull ? INT64_MAX : (long long)best;
+  r.ant = best == ~0ull ? -1 : (int)ant;
+  r.feasible = feas;
+  return r;
+}
+
 /* Iteration best over feasible ants (lowest cost, ties to the lower ant) and
  * the global-best / stagnation bookkeeping of aco_cuda.cu:104-126,331-399.
  * Returns (to every thread) whether a feasible ant was found. */
 __device__ bool select_best_block(const DevCtx &P, int colony) {
-  __shared__ unsigned long long s_min[16];
-  __shared__ int s_cnt[16];
   __shared__ int s_copy, s_found;
   AcoHipColonyState *S = P.state + colony;
   const ColonyDesc D = P.col[colony];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  unsigned long long key = ~0ull;
-  int cnt = 0;
-  for (int a = threadIdx.x; a < D.m; a += blockDim.x) {
-    if (P.unvis[D.ant_off + a] == 0) {
-      const unsigned long long kk =
-          ((unsigned long long)P.cost[D.ant_off + a] << 20) | (unsigned long long)a;
-      key = kk < key ? kk : key;
-      ++cnt;
-    }
-  }
-  key = wave_min_u64(key);
-  cnt = (int)wave_sum_i64(cnt);
-  if (lane == 0) {
-    s_min[wave] = key;
-    s_cnt[wave] = cnt;
-  }
-  __syncthreads();
+  const BestAnt bb = block_best_ant(P, D, false);
   if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
-      if (s_min[w] < s_min[0]) s_min[0] = s_min[w];
-      s_cnt[0] += s_cnt[w];
-    }
-    const unsigned long long best = s_min[0];
-    S->feasible_ants = s_cnt[0];
+    S->feasible_ants = bb.feasible;
     S->improved = 0;
     s_copy = -1;
-    if (best != ~0ull) {
-      const int ant = (int)(best & 0xFFFFFu);
-      const long long c = (long long)(best >> 20);
+    if (bb.ant >= 0) {
+      const int ant = bb.ant;
+      const long long c = bb.cost;
       S->found = 1;
       S->best_ant = ant;
       S->iter_best_cost = c;
@@ -999,8 +1035,8 @@ __device__ bool select_best_block(const DevCtx &P, int colony) {
         S->iter_since_best++;
       }
       const float qeff = P.qdep * 100.0f;
-      S->dep_iter = (0.3f * qeff) / (float)c;
-      S->dep_best = (0.7f * qeff) / (float)S->global_best_cost;
+      S->dep_iter = (0.3f * qeff) / cost_len(P.dist_real, c);
+      S->dep_best = (0.7f * qeff) / cost_len(P.dist_real, S->global_best_cost);
     } else {
       S->found = 0;
       S->best_ant = -1;
@@ -1037,8 +1073,8 @@ __device__ __forceinline__ uint32_t dep_new(const UpdTables &T, uint32_t q, floa
 
 /* Patch tau_cand and score of row u for candidate v after tau[u][v] became
  * `val`.  slot_of gives the slot in the same memory round trip as the
- * pheromone bytes; d(u,v) comes from the coordinates (the kNN stored the same
- * nint distance).  Rows with padding slots (k >= n, index 0 repeated) patch
+ * pheromone bytes; the distance code d of (u,v) comes from the coordinates
+ * (the kNN stored the same code).  Rows with padding slots (k >= n, index 0 repeated) patch
  * every matching slot by scanning the row. */
 __device__ __forceinline__ void patch_slot(const DevCtx &P, const UpdTables &T,
                                            const InstDesc &I, uint8_t *__restrict__ tc,
@@ -1118,8 +1154,7 @@ __device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDe
     const uint32_t b = tau[(size_t)v * S + u];
     const int su = so[(size_t)u * n1 + v];
     const int sv = so[(size_t)v * n1 + u];
-    const int2 pu = xy[u], pv = xy[v];
-    const int d = aco_nint_dist(pu.x, pu.y, pv.x, pv.y);
+    const int d = (int)edge_code(P, xy[u], xy[v]);
     uint32_t tuv, tvu;
     if (u == 0 && vp1 == 0) {
       /* single-customer route 0 -> v -> 0: both edges in order */
@@ -1207,8 +1242,7 @@ __device__ __forceinline__ void pair_apply(const UpdTables &T, bool fwd_x, float
 __global__ void __launch_bounds__(256) k_select_deposit(DevCtx P, int stage_eta,
                                                         const int *cmap) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dyn_smem[];
-  __shared__ unsigned long long s_min[4];
-  __shared__ int s_cnt[4], s_last, s_zz[2];
+  __shared__ int s_last, s_zz[2];
   __shared__ float s_deq[256], s_thr[256], s_tp[256];
   /* grid row y: colony y, or colony cmap[y] of a size bucket's list */
   const int colony = cmap ? cmap[blockIdx.y] : (int)blockIdx.y;
@@ -1221,45 +1255,22 @@ __global__ void __launch_bounds__(256) k_select_deposit(DevCtx P, int stage_eta,
   const int best_len0 = S->best_len;
 
   /* iteration best: feasible ants only, lowest cost, ties to the lower ant */
-  unsigned long long key = ~0ull;
-  int cnt = 0;
-  if (!halted) {
-    for (int a = threadIdx.x; a < D.m; a += nt) {
-      if (P.unvis[D.ant_off + a] == 0) {
-        const unsigned long long kk =
-            ((unsigned long long)P.cost[D.ant_off + a] << 20) | (unsigned long long)a;
-        key = kk < key ? kk : key;
-        ++cnt;
-      }
-    }
-  }
-  key = wave_min_u64(key);
-  cnt = (int)wave_sum_i64(cnt);
-  if (lane == 0) {
-    s_min[wave] = key;
-    s_cnt[wave] = cnt;
-  }
   for (int i = threadIdx.x; i < 256; i += nt) {
     s_deq[i] = P.deq[i];
     s_thr[i] = i < 255 ? P.thr[i] : 0.0f;
     s_tp[i] = P.taupow[i];
   }
   if (threadIdx.x < 2) s_zz[threadIdx.x] = 0;
-  __syncthreads();
-  unsigned long long best = s_min[0];
-  int feasible = s_cnt[0];
-  for (int w = 1; w < (nt >> 6); ++w) {
-    best = s_min[w] < best ? s_min[w] : best;
-    feasible += s_cnt[w];
-  }
-  const bool found = best != ~0ull;
-  const int ant = found ? (int)(best & 0xFFFFFu) : -1;
-  const long long c = found ? (long long)(best >> 20) : INT64_MAX;
+  const BestAnt bb = block_best_ant(P, D, halted); /* its barriers order the stores above */
+  const int feasible = bb.feasible;
+  const bool found = bb.ant >= 0;
+  const int ant = bb.ant;
+  const long long c = bb.cost;
   const bool improved = found && c < gbest;
   const long long gnew = improved ? c : gbest;
   const float qeff = P.qdep * 100.0f;
-  const float delta1 = found ? (0.3f * qeff) / (float)c : 0.0f;
-  const float delta2 = found ? (0.7f * qeff) / (float)gnew : 0.0f;
+  const float delta1 = found ? (0.3f * qeff) / cost_len(P.dist_real, c) : 0.0f;
+  const float delta2 = found ? (0.7f * qeff) / cost_len(P.dist_real, gnew) : 0.0f;
   const int32_t *g1 = found ? P.tours + D.tour_off + (size_t)ant * D.stride : nullptr;
   const int len1 = found ? P.len[D.ant_off + ant] : 0;
   const int32_t *g2 = improved ? g1 : P.best + D.best_off;
@@ -1337,7 +1348,7 @@ __global__ void __launch_bounds__(256) k_select_deposit(DevCtx P, int stage_eta,
       const int2 px = xy[x], py = xy[y];
       for (int e = 0; e < n1e; ++e) pair_apply(T, f1[e] == x, delta1, A, B);
       for (int e = 0; e < n2e; ++e) pair_apply(T, f2[e] == x, delta2, A, B);
-      const int d = aco_nint_dist(px.x, px.y, py.x, py.y);
+      const int d = (int)edge_code(P, px, py);
       tau[xy_] = (uint8_t)A;
       tau[yx_] = (uint8_t)B;
       patch_slot(P, T, I, tc, score, x, y, sxy, d, (uint8_t)A);
@@ -1542,8 +1553,8 @@ __global__ void __launch_bounds__(1024) k_deposit_tour(DevCtx P, int colony, con
 
 /* ---- route-cost reduction ------------------------------------------------- */
 
-/* One wavefront per tour: int64 sum of nint edge lengths (the reference adds
- * calc_dist per move in-kernel, aco_cuda_kernels.cu:540,550). */
+/* One wavefront per tour: int64 sum of its edge costs in cost units (the
+ * reference adds calc_dist per move in-kernel, aco_cuda_kernels.cu:540,550). */
 __global__ void __launch_bounds__(256) k_route_costs(DevCtx P, int instance, const int32_t *tours,
                                                       const int32_t *lens, int count, int stride,
                                                       long long *out) {
@@ -1556,8 +1567,7 @@ __global__ void __launch_bounds__(256) k_route_costs(DevCtx P, int instance, con
   const int len = lens[t];
   long long acc = 0;
   for (int p = lane; p + 1 < len; p += 64) {
-    const int2 a = xy[tour[p]], b = xy[tour[p + 1]];
-    acc += aco_nint_dist(a.x, a.y, b.x, b.y);
+    acc += edge_units(P, xy[tour[p]], xy[tour[p + 1]]);
   }
   acc = wave_sum_i64(acc);
   if (lane == 0) out[t] = acc;

@@ -128,9 +128,12 @@ __global__ void __launch_bounds__(256) k_edge_features(DevCtx P, int inst, uint1
   if (e >= slots) return;
   const int i = (int)(e / k), t = (int)(e - (long)i * k);
   const uint32_t r = P.rec[I.cand_off + e];
-  const int j = (int)(r & 0xFFFFu), d = (int)(r >> 16);
+  const int j = (int)(r & 0xFFFFu);
   const int2 *xy = P.xy + I.node_off;
   const int2 p0 = xy[0], pi = xy[i], pj = xy[j];
+  /* nint distance in either distance mode; 0 on padding slots (index 0,
+   * which no candidate list holds otherwise) */
+  const int d = j ? aco_nint_dist(pi.x, pi.y, pj.x, pj.y) : 0;
   const float L = I.scale;
   float f[MLP_F];
   f[0] = (float)(pi.x - p0.x) / L;

@@ -6,9 +6,10 @@
  * with the ACO_SOLVER_* environment knobs, the VEHICLES mismatch check, the
  * final validation, and the "Route i: ...", "Cost: %.3f", "best cost: %.3f"
  * output.  alpha=1, beta=2, rho=0.5, tau0=1, Q=1 as in the reference
- * (main_vrp.cu:53-57).  Differences: weighted DEMAND_SECTION is accepted and
- * capacity-checked; the reported cost is the integer TSPLIB (nint) cost the
- * solver optimises, or the unrounded cost with ACO_HIP_COST=real.
+ * (main_vrp.cu:53-57).  Like the reference it optimises and prints the
+ * unrounded route length (instance_parser.c:17-19); ACO_HIP_DIST=nint
+ * optimises and prints the integer TSPLIB (nint) length instead.  Weighted
+ * DEMAND_SECTION is accepted and capacity-checked.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -53,10 +54,14 @@ int main(int argc, char **argv) {
     vrp_free(&inst);
     return 1;
   }
+  const char *dist = getenv("ACO_HIP_DIST");
+  const int real = !(dist && strcmp(dist, "nint") == 0);
   int32_t len = 0;
   int64_t cost = 0;
   int status = aco_vrp_hip_with_capacity(n, K, inst.capacity, m, inst.x, inst.y, inst.demand, 1.0,
-                                         2.0, 0.5, 1.0, 1.0, seed, tour, &len, &cost);
+                                         2.0, 0.5, 1.0, 1.0, seed,
+                                         real ? ACO_HIP_DIST_REAL : ACO_HIP_DIST_NINT, tour, &len,
+                                         &cost);
   int rc = 0;
   if (status != ACO_HIP_OK) {
     fprintf(stderr, "HIP solver failed: %s\n", aco_status_string(status));
@@ -65,16 +70,18 @@ int main(int argc, char **argv) {
     fprintf(stderr, "invalid solution: tour does not hold %d routes\n", K);
     rc = 1;
   } else {
-    const char *mode = getenv("ACO_HIP_COST");
-    double printed = (mode && strcmp(mode, "real") == 0) ? solution_cost_real(best, &inst)
-                                                         : (double)cost;
+    /* the cost of the metric optimised: the exact sum of the float edge
+     * lengths (real) or the integer length (nint) */
+    const double printed = real ? (double)cost * ACO_HIP_REAL_COST_UNIT : (double)cost;
+    const int64_t check = real ? solution_cost_real_units(best, &inst)
+                               : solution_cost_nint(best, &inst);
     char err[160];
     if (!cli_validate_solution_or_report(best, n, K, printed)) {
       rc = 1;
     } else if (!solution_validate_capacity(best, &inst, err, sizeof(err))) {
       fprintf(stderr, "invalid solution: %s\n", err);
       rc = 1;
-    } else if (solution_cost_nint(best, &inst) != cost) {
+    } else if (check != cost) {
       fprintf(stderr, "invalid solution: cost mismatch\n");
       rc = 1;
     } else {

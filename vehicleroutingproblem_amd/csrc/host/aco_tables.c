@@ -54,6 +54,25 @@ void aco_eta_table_build(float *eta, int dmax, float beta) {
   }
 }
 
+/* eta^beta in real mode, by heuristic-table index (aco_real_eta_bin):
+ * index 0 is d = 0, which uses 1e-9f like 1/(0 + CUDA_EPS); index b >= 1 is
+ * the middle float c of its bin, eta = c^-beta in double, rounded once.
+ * The reference evaluates powf(1/(d + 1e-9f), beta) of the float distance
+ * (aco_cuda_kernels.cu:207-212) and rsqrtf(d^2 + 1e-9f)^beta in the fallback
+ * (:400-406); within a bin of relative width 2^-10 the table value is within
+ * beta * 2^-11 (relative) of d^-beta. */
+void aco_eta_table_build_real(float *eta, int len, float beta) {
+  for (int b = 0; b < len; ++b) {
+    const double base = b == 0 ? (double)1e-9f : (double)aco_real_bin_centre((uint32_t)b);
+    eta[b] = (float)pow(base, -(double)beta);
+  }
+}
+
+/* Real distances of squared lengths (the arithmetic of the kernels). */
+void aco_real_dist_batch(const uint32_t *s, int64_t count, float *out) {
+  for (int64_t i = 0; i < count; ++i) out[i] = aco_real_dist_s(s[i]);
+}
+
 void aco_philox_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   aco_philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
 }

@@ -6,6 +6,7 @@
  * leaves to offline pyvrp validation (tools/python/validate_pyvrp.py:30-107).
  */
 #include "aco_host.h"
+#include "aco_inline.h"
 
 #include <errno.h>
 #include <float.h>
@@ -164,6 +165,21 @@ int64_t solution_cost_nint(const Solution *s, const VrpInstance *inst) {
     for (int t = 0; t + 1 < r->len; ++t) {
       int u = r->nodes[t], v = r->nodes[t + 1];
       cost += vrp_nint_dist(inst->x[u], inst->y[u], inst->x[v], inst->y[v]);
+    }
+  }
+  return cost;
+}
+
+/* Real-mode route cost in the solver's units: the sum over edges of the
+ * float distance times 2^23 (exact; aco_real_units). */
+int64_t solution_cost_real_units(const Solution *s, const VrpInstance *inst) {
+  int64_t cost = 0;
+  for (int i = 0; i < s->K; ++i) {
+    const Route *r = &s->routes[i];
+    for (int t = 0; t + 1 < r->len; ++t) {
+      int u = r->nodes[t], v = r->nodes[t + 1];
+      int32_t dx = inst->x[u] - inst->x[v], dy = inst->y[u] - inst->y[v];
+      cost += aco_real_units(aco_real_dist_s((uint32_t)(dx * dx) + (uint32_t)(dy * dy)));
     }
   }
   return cost;

@@ -129,12 +129,13 @@ class Params:
     ants_per_group: int = 0
     stage_lds: int = -1
     fallback_matrix: int = -1
+    dist_mode: int = N.DIST_NINT
 
     def to_c(self) -> N.AcoHipParams:
         p = N.AcoHipParams()
         for f in ("cand_k", "ants", "colonies", "alpha", "beta", "rho", "tau0", "Q",
                   "depot_close_weight", "knn_key", "ants_per_group", "stage_lds",
-                  "fallback_matrix"):
+                  "fallback_matrix", "dist_mode"):
             setattr(p, f, getattr(self, f))
         return p
 
@@ -419,9 +420,10 @@ class HipSolver:
 
 
 def solve(instance: Instance, ants: int = 256, seed: int = 1234, alpha=1.0, beta=2.0, rho=0.5,
-          tau0=1.0, Q=1.0):
+          tau0=1.0, Q=1.0, dist_mode: int = N.DIST_NINT):
     """aco_vrp_with_capacity (include/aco.h:127-131) on the GPU: returns
-    (best tour, integer cost).  Stop rules come from ACO_SOLVER_*."""
+    (best tour, cost in cost units: the nint length, or the real length
+    times 2^23 with dist_mode=DIST_REAL).  Stop rules come from ACO_SOLVER_*."""
     lib = N.load()
     tour = np.zeros(instance.n + instance.K + 1 + 64, np.int32)
     ln = C.c_int32()
@@ -429,11 +431,21 @@ def solve(instance: Instance, ants: int = 256, seed: int = 1234, alpha=1.0, beta
     rc = lib.aco_vrp_hip_with_capacity(instance.n, instance.K, instance.capacity, ants,
                                        N.ptr(instance.x, C.c_int32), N.ptr(instance.y, C.c_int32),
                                        N.ptr(instance.demand, C.c_int32), alpha, beta, rho, tau0,
-                                       Q, seed, N.ptr(tour, C.c_int32), C.byref(ln),
+                                       Q, seed, dist_mode, N.ptr(tour, C.c_int32), C.byref(ln),
                                        C.byref(cost))
     if rc:
         raise AcoError(rc, lib.aco_status_string(rc).decode())
     return tour[:ln.value].copy(), int(cost.value)
+
+
+def real_dist(s) -> np.ndarray:
+    """The float nearest to sqrt(s) for squared lengths s (uint32), with the
+    kernels' arithmetic (host build of the same inline code)."""
+    s = np.ascontiguousarray(s, np.uint32)
+    out = np.zeros(s.shape, np.float32)
+    N.load().aco_real_dist_batch(s.ctypes.data_as(C.POINTER(C.c_uint32)), s.size,
+                                 N.ptr(out, C.c_float))
+    return out
 
 
 def split_routes(tour) -> list:
