@@ -336,9 +336,11 @@ def quality_leg():
     50-epoch stagnation stop in the build container; its best cost and epoch
     count are the fixture tests/golden/ref_seq_quality.json.  This solver
     runs exactly as many epochs with the same parameters (alpha 1, beta 2,
-    rho 0.5, tau0 1, Q 1, k=32) and its best tour is costed in real (double)
-    distances like the reference's print."""
+    rho 0.5, tau0 1, Q 1, k=32) on the same metric (real-distance mode) and
+    its best tour is costed in real (double) distances like the reference's
+    print."""
     from vehicleroutingproblem_amd import HipSolver, Instance, Params
+    from vehicleroutingproblem_amd import _native as N
     path = os.path.join(ROOT, "tests", "golden", "ref_seq_quality.json")
     if not os.path.exists(path):
         return None
@@ -350,7 +352,8 @@ def quality_leg():
         inst = Instance.uniform(100, 8, side=1000, dmin=1, dmax=1, slack_pct=120,
                                 seed=r["instance_seed"])
         with HipSolver(inst, Params(cand_k=32, ants=doc["ants"], alpha=1.0, beta=2.0, rho=0.5,
-                                    tau0=1.0, Q=1.0), seeds=[r["solver_seed"]]) as s:
+                                    tau0=1.0, Q=1.0, dist_mode=N.DIST_REAL),
+                       seeds=[r["solver_seed"]]) as s:
             s.prepare()
             s.run(fixed_epochs=r["epochs"], chunk_epochs=div_chunk(r["epochs"], 64))
             tour, _ = s.best()
@@ -362,7 +365,7 @@ def quality_leg():
     return {"workload": f"{len(runs)} unit-demand uniform instances, n=100, K=8, "
                         f"Q=ceil(1.2n/8), {doc['ants']} ants; reference sequential solver to its "
                         "stagnation stop (50), recorded in the build container; this solver for "
-                        "the same number of epochs",
+                        "the same number of epochs, both on real (unrounded) distances",
             "mean_best_cost_diff_pct": round(100 * sum(diff) / len(diff), 3),
             "all_tours_feasible": feasible,
             "mean_epochs": round(sum(epochs) / len(epochs), 1),
@@ -377,8 +380,8 @@ def reference_campaign_leg(budget_s: float = 2.0):
     campaign runner: all six runs in one batched context under one wall-clock
     budget (ACO_SOLVER_TIMEOUT_SECONDS), against the best costs and per-run
     elapsed times that campaign recorded
-    (tests/golden/ref_cuda_backend_sizes_n500_n1000.csv).  Costs are real
-    (unrounded) route lengths on both sides."""
+    (tests/golden/ref_cuda_backend_sizes_n500_n1000.csv).  Both sides
+    optimise and report real (unrounded) route lengths."""
     import csv
     import tempfile
     from vehicleroutingproblem_amd import campaign
@@ -386,7 +389,7 @@ def reference_campaign_leg(budget_s: float = 2.0):
     with open(os.path.join(golden, "ref_cuda_backend_sizes_n500_n1000.csv"), newline="") as f:
         recorded = {(r["name"], int(r["run_id"])): r for r in csv.DictReader(f)}
     keys = ("ACO_SOLVER_TIMEOUT_SECONDS", "ACO_SOLVER_STAGNATION_EPOCHS",
-            "ACO_SOLVER_FIXED_EPOCHS", "ACO_HIP_CAND_K")
+            "ACO_SOLVER_FIXED_EPOCHS", "ACO_HIP_CAND_K", "ACO_HIP_DIST")
     saved = {k: os.environ.get(k) for k in keys}
     try:
         for k in keys:
@@ -419,8 +422,9 @@ def reference_campaign_leg(budget_s: float = 2.0):
                      "reference_cuda_best_cost": float(ref["best_cost"]),
                      "reference_cuda_elapsed_s": float(ref["elapsed_s"])})
     return {"workload": "reference instances n500_k8_s19000 and n1000_k8_s19001 (unit demand, "
-                        "K=8), m=256, seeds 1234-1236, k=32, rho=0.5: the reference CUDA "
-                        "campaign's recorded runs against one batched solve of all six",
+                        "K=8), m=256, seeds 1234-1236, k=32, rho=0.5, real distances: the "
+                        "reference CUDA campaign's recorded runs against one batched solve of "
+                        "all six",
             "budget_s_all_six_runs": budget_s,
             "batch_elapsed_s": float(rows[0]["elapsed_s"]) if rows and rows[0]["elapsed_s"] else None,
             "all_ok": all(r["status"] == "ok" for r in rows),

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from vehicleroutingproblem_amd import HipSolver, Instance, Params
+from vehicleroutingproblem_amd import _native as N
 from vehicleroutingproblem_amd import campaign as CP
 from vehicleroutingproblem_amd.solver import split_routes
 
@@ -18,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 def test_campaign_solve(gpu, tmp_path, monkeypatch):
     monkeypatch.setenv("ACO_SOLVER_FIXED_EPOCHS", "6")
-    for k in ("ACO_SOLVER_STAGNATION_EPOCHS", "ACO_SOLVER_TIMEOUT_SECONDS", "ACO_HIP_CAND_K"):
+    for k in ("ACO_SOLVER_STAGNATION_EPOCHS", "ACO_SOLVER_TIMEOUT_SECONDS", "ACO_HIP_CAND_K",
+              "ACO_HIP_DIST"):
         monkeypatch.delenv(k, raising=False)
     manifest = CP.generate(str(tmp_path / "inst"), [25, 70, 140], seeds=1, first_seed=900, m=24,
                            solver_seed=55)
@@ -40,9 +42,11 @@ def test_campaign_solve(gpu, tmp_path, monkeypatch):
         assert res["ok"], res
         assert float(r["best_cost"]) == printed
         assert r["solver_seed"] == mrows[r["name"]]["solver_seed"] + r["run_id"] - 1
-        # the same run alone: same seed, settings and epochs give the same best tour
+        # the same run alone: same seed, settings, metric (real, the campaign's
+        # default) and epochs give the same best tour
         with HipSolver(inst, Params(cand_k=min(32, inst.n), ants=24, alpha=1.0, beta=2.0,
-                                    rho=0.5, tau0=1.0, Q=1.0), seeds=[r["solver_seed"]]) as s:
+                                    rho=0.5, tau0=1.0, Q=1.0, dist_mode=N.DIST_REAL),
+                       seeds=[r["solver_seed"]]) as s:
             s.prepare()
             s.run(fixed_epochs=6)
             tour, _ = s.best()

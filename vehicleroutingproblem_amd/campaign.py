@@ -10,7 +10,9 @@
   the same ``m`` go into ONE batched context (one colony per run, launched by
   size bucket), instead of one process per run.  The solver settings are the
   CLI's (alpha 1, beta 2, rho 0.5, tau0 1, Q 1; k = min(ACO_HIP_CAND_K or 32,
-  n); stop rules from ACO_SOLVER_*, 100 stagnant epochs when none is set).
+  n); stop rules from ACO_SOLVER_*, 100 stagnant epochs when none is set),
+  optimising the reference's unrounded length (ACO_HIP_DIST=nint optimises
+  the integer TSPLIB length instead).
   Each run writes a solution file in the CLI's text format ("Route i: ..."
   per vehicle, then "Cost: %.3f" and "best cost: %.3f", cli_common.c:49-68;
   the real, unrounded cost like the reference's print) and a CSV row in the
@@ -205,10 +207,15 @@ def _runtime_config():
 
 
 def solve(manifest: str, csv_path: str, solutions_dir: str, repeats: int = 1, root: str = None,
-          m_override: int = 0, backend: str = "hip", device: int = 0) -> list:
+          m_override: int = 0, backend: str = "hip", device: int = 0,
+          dist_mode: int = None) -> list:
     """Solves every row x repeats; writes the solution files and the CSV
-    (rewritten whole).  Returns the CSV rows."""
+    (rewritten whole).  Returns the CSV rows.  dist_mode: the metric
+    optimised (default: real unless ACO_HIP_DIST=nint)."""
+    from . import _native as N
     from .solver import AcoError, HipSolver, Instance, Params, split_routes
+    if dist_mode is None:
+        dist_mode = N.DIST_NINT if os.environ.get("ACO_HIP_DIST") == "nint" else N.DIST_REAL
     rows = read_manifest(manifest)
     root = os.path.abspath(root or os.path.dirname(os.path.abspath(manifest)))
     os.makedirs(solutions_dir, exist_ok=True)
@@ -240,7 +247,7 @@ def solve(manifest: str, csv_path: str, solutions_dir: str, repeats: int = 1, ro
         status, err = "ok", ""
         try:
             with HipSolver(g_insts, Params(cand_k=32, ants=m, alpha=1.0, beta=2.0, rho=0.5,
-                                           tau0=1.0, Q=1.0),
+                                           tau0=1.0, Q=1.0, dist_mode=dist_mode),
                            seeds=[x[2] for x in group],
                            cand_k=[min(kcap, inst.n) for inst in g_insts], device=device) as s:
                 s.prepare()
