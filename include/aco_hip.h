@@ -259,7 +259,9 @@ int aco_hip_set_epoch(AcoHipContext *ctx, int colony, int epoch);
 int aco_hip_get_best(AcoHipContext *ctx, int colony, int32_t *tour, int32_t *len,
                      int64_t *cost);
 /* Deposit a host tour on one colony (sequential edge semantics of
- * kernel_deposit_flat_solution, aco_cuda_kernels.cu:637-668). */
+ * kernel_deposit_flat_solution, aco_cuda_kernels.cu:637-668).  The tour
+ * starts and ends at the depot and visits each customer at most once; other
+ * tours are refused with ACO_HIP_ERR_INVALID_INPUT. */
 int aco_hip_deposit_tour(AcoHipContext *ctx, int colony, const int32_t *tour, int len,
                          float delta);
 /* Route-cost reduction of host tours of instance `instance` (int64 nint sums);

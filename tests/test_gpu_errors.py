@@ -38,7 +38,9 @@ def _set(a, i, v):
 
 BAD_INSTANCES = {
     "depot_only": (lambda b: Instance([500], [500], [0], 1, 10), "1 <= n <= 65534"),
-    "no_vehicles": (lambda b: _with(b, K=0), "K >= 1"),
+    "no_vehicles": (lambda b: _with(b, K=0), "1 <= K <= 65535"),
+    # (K-1)*Q would overflow the device's int32 remaining-capacity test
+    "too_many_vehicles": (lambda b: _with(b, K=70000), "1 <= K <= 65535"),
     "zero_capacity": (lambda b: _with(b, capacity=0), "1 <= capacity <= 32767"),
     "capacity_too_large": (lambda b: _with(b, capacity=40000), "1 <= capacity <= 32767"),
     "x_out_of_range": (lambda b: _with(b, x=_set(b.x, 3, 40000)), "coordinates"),
@@ -66,6 +68,8 @@ BAD_PARAMS = {
     "rho_1": (dict(rho=1.0), "rho"),
     "tau0_0": (dict(tau0=0.0), "tau0"),
     "colonies_0": (dict(colonies=0), "colonies"),
+    "colonies_too_many": (dict(colonies=(1 << 20) + 1), "colonies"),
+    "dist_mode_2": (dict(dist_mode=2), "dist_mode"),
 }
 
 
@@ -130,6 +134,12 @@ def test_bad_indices_and_tours(gpu):
         with pytest.raises(AcoError) as e:  # too short
             s.deposit_tour([0], 0.5)
         assert e.value.status == N.ACO_HIP_ERR_INVALID_INPUT
+        tau0 = s.tau()[0].copy()
+        for bad in ([0, 3, 5, 3, 0], [0, 2, 4, 2, 0, 0], [1, 2, 0], [0, 2, 4]):
+            with pytest.raises(AcoError) as e:  # repeated customer / not depot-delimited
+                s.deposit_tour(bad, 0.5)
+            assert e.value.status == N.ACO_HIP_ERR_INVALID_INPUT, bad
+        assert np.array_equal(s.tau()[0], tau0)  # nothing was deposited
         tours = np.zeros((2, 8), np.int32)
         with pytest.raises(AcoError) as e:  # a length past the tour stride
             s.route_costs(tours, np.array([2, 9], np.int32))

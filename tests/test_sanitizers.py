@@ -53,6 +53,10 @@ def test_vrp_io_clean_under_sanitizers(tmp_path):
         "bigdemand.vrp": "NAME: x\nTYPE: CVRP\nDIMENSION: 3\nVEHICLES: 1\nCAPACITY: 5\n"
                          "NODE_COORD_SECTION\n1 0 0\n2 1 1\n3 2 2\nDEMAND_SECTION\n1 0\n2 1\n"
                          "3 99999999999\nEOF\n",
+        # a second DIMENSION after the coordinates were read with the first
+        "redim.vrp": "NAME: x\nTYPE: CVRP\nDIMENSION: 3\nVEHICLES: 1\nCAPACITY: 5\n"
+                     "NODE_COORD_SECTION\n1 0 0\n2 1 1\n3 2 2\nDIMENSION: 9\n"
+                     "DEMAND_SECTION\n1 0\n2 1\n3 1\n4 1\n5 1\n6 1\n7 1\n8 1\n9 1\nEOF\n",
     }
     paths = []
     for name, text in bad.items():

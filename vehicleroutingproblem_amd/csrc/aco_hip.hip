@@ -272,6 +272,11 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if (p->ants < 1 || p->ants > (1 << 20))
     return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "ants must be in [1, 2^20]");
   if (p->colonies < 1) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "colonies must be >= 1");
+  /* colony indices, per-colony grids and state arrays are int-sized */
+  if ((long long)num * p->colonies > (1LL << 20))
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
+                   "instances x colonies must be at most 2^20 (got %lld)",
+                   (long long)num * p->colonies);
   if (!(p->rho > 0.0f && p->rho < 1.0f))
     return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "rho must be in (0, 1)");
   if (!(p->tau0 > 0.0f)) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tau0 must be > 0");
@@ -292,10 +297,13 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   int row_base = 0;
   for (int i = 0; i < num; ++i) {
     const AcoHipInstance &a = in[i];
-    if (a.n < 1 || a.n > 65534 || a.K < 1 || a.capacity < 1 || a.capacity > 32767 || !a.x ||
-        !a.y || !a.demand)
+    /* K <= 65535 keeps the close rule's remaining capacity (K-v-1)*Q within
+     * int32 on the device (65534 * 32767 < 2^31) */
+    if (a.n < 1 || a.n > 65534 || a.K < 1 || a.K > 65535 || a.capacity < 1 ||
+        a.capacity > 32767 || !a.x || !a.y || !a.demand)
       return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
-                     "instance %d: need 1 <= n <= 65534, K >= 1, 1 <= capacity <= 32767", i);
+                     "instance %d: need 1 <= n <= 65534, 1 <= K <= 65535, "
+                     "1 <= capacity <= 32767", i);
     if (a.demand[0] != 0)
       return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "instance %d: depot demand must be 0", i);
     int32_t xmin = a.x[0], xmax = a.x[0], ymin = a.y[0], ymax = a.y[0];
@@ -1387,6 +1395,20 @@ extern "C" int aco_hip_deposit_tour(AcoHipContext *c, int colony, const int32_t 
   for (int p = 0; p < len; ++p)
     if (tour[p] < 0 || tour[p] > d.n)
       return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tour node out of range at %d", p);
+  /* the kernel gives each edge its own thread: only a depot-delimited tour
+   * visiting each customer at most once has no other shared entries than
+   * those it orders itself (single-customer and empty routes) */
+  if (tour[0] != 0 || tour[len - 1] != 0)
+    return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "tour must start and end at the depot");
+  {
+    std::vector<unsigned char> seen((size_t)d.n + 1, 0);
+    for (int p = 0; p < len; ++p) {
+      const int v = tour[p];
+      if (v == 0) continue;
+      if (seen[v]) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "customer %d repeats at %d", v, p);
+      seen[v] = 1;
+    }
+  }
   HIPC(c, hipMemcpyAsync(c->d_scratch_tour, tour, sizeof(int32_t) * len, hipMemcpyHostToDevice,
                          c->stream));
   hipLaunchKernelGGL(k_deposit_tour, dim3(1), dim3(1024), 0, c->stream, c->P, colony,

@@ -88,7 +88,14 @@ int vrp_read(const char *path, VrpInstance *out, int require_unit_demand) {
       keyword_str(line, 4, name, sizeof(name));
     } else if (strncmp(line, "DIMENSION", 9) == 0) {
       int dim = 0;
-      if (keyword_int(line, 9, &dim)) n = dim - 1;
+      if (keyword_int(line, 9, &dim)) {
+        /* the sections already read were sized by the earlier DIMENSION */
+        if ((x || dem) && dim - 1 != n) {
+          fprintf(stderr, "DIMENSION changes after a section was read\n");
+          goto done;
+        }
+        n = dim - 1;
+      }
     } else if (strncmp(line, "VEHICLES", 8) == 0) {
       keyword_int(line, 8, &K);
     } else if (strncmp(line, "CAPACITY", 8) == 0) {
