@@ -32,13 +32,15 @@ HBM_PEAK_GBS = 8000.0
 # seeds in the launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
 # this bench, each scaled by what the counter reads for a 256 MiB device copy
 # (fetch x2.0, write x1.0); the commands and figures are in
-# profiles/k_construct_n1000_m256_hbm_traffic.md.
-CONSTRUCT_TRAFFIC_BYTES = {1: 20869168}
+# profiles/k_construct_n1000_m256_hbm_traffic.md (1 seed) and
+# profiles/k_construct_n1000_32seeds_hbm_traffic.md (32 seeds).
+CONSTRUCT_TRAFFIC_BYTES = {1: 20869168, 32: 310561551}
 # Instructions k_construct issues per decision at this workload (VALU + SALU +
-# LDS + branch SQ counters over decisions, profiles/k_construct_n1000_m256_sq_counters.md):
-# a lone wavefront issues at most one instruction per 4-cycle slot of its SIMD,
+# LDS + branch SQ counters over decisions; profiles/k_construct_n1000_m256_sq_counters.md
+# for one seed, profiles/k_construct_n1000_32seeds_sq_counters.md for 32): a
+# lone wavefront issues at most one instruction per 4-cycle slot of its SIMD,
 # so 4 x this is the issue bound of one decision in shader cycles.
-CONSTRUCT_INSTS_PER_DECISION = 103.0
+CONSTRUCT_INSTS_PER_DECISION = {1: 103.0, 32: 102.1}
 
 
 def tour_bytes(n, K, k):
@@ -524,8 +526,10 @@ def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int =
             "fallback_cycle_share": round(dev["fallback_cycles"] / max(1, dev["ant_cycles"]), 4),
             "cycles_per_decision": round(cpd, 1),
             "cycles_per_ant": round(dev["ant_cycles"] / max(1, dev["ants"])),
-            "issue_bound_cycles_per_decision": 4 * CONSTRUCT_INSTS_PER_DECISION,
-            "issue_bound_frac": round(4 * CONSTRUCT_INSTS_PER_DECISION / max(1e-9, cpd), 3)},
+            "insts_per_decision": CONSTRUCT_INSTS_PER_DECISION.get(len(seeds)),
+            "issue_bound_cycles_per_decision": (
+                4 * CONSTRUCT_INSTS_PER_DECISION[len(seeds)]
+                if len(seeds) in CONSTRUCT_INSTS_PER_DECISION else None)},
         "update_phases_us": ({
             k: round(1e6 * dev[k + "_ticks"] / max(1, dev["wall_clock_hz"]) / dev["updates"], 2)
             for k in ("evap", "select", "deposit")} if dev["updates"] else None),
