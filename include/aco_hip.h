@@ -183,16 +183,24 @@ typedef struct {
   int stage_lds;         /* candidate rows staged in LDS (1) or read from global memory (0) */
   int lds_bytes;         /* LDS per workgroup */
   float tuned_ms;        /* construct time of the chosen shape (autotuner), 0 = not tuned */
+  int ants_per_wave;     /* 1, or 2: two ants per wavefront (candidate lists <= 32) */
 } AcoHipBucketInfo;
 int aco_hip_num_buckets(const AcoHipContext *ctx);
 int aco_hip_get_bucket(const AcoHipContext *ctx, int bucket, AcoHipBucketInfo *info);
 /* Force a bucket's shape (stage_lds -1 = automatic); the effective shape may
- * have fewer ants per workgroup when LDS does not fit the request. */
+ * have fewer ants per workgroup when LDS does not fit the request.  Ants per
+ * wavefront stay automatic. */
 int aco_hip_set_bucket_shape(AcoHipContext *ctx, int bucket, int ants_per_group, int stage_lds);
-/* Time `launches` construct launches of every bucket for each ants-per-
- * workgroup in {1, 2, 4, 8} and each staging mode that fits, apply the
- * fastest per bucket and report them (info: max_buckets entries).  Colonies
- * must not be halted; pheromone and colony state are left unchanged. */
+/* The same with the ants per wavefront: 0 automatic, 1, or 2 (two ants per
+ * wavefront, for candidate lists of at most 32; 1 is used above that).  With
+ * two, ants_per_group counts wavefronts, each holding two ants. */
+int aco_hip_set_bucket_layout(AcoHipContext *ctx, int bucket, int ants_per_group, int stage_lds,
+                              int ants_per_wave);
+/* Time `launches` construct launches of every bucket for each wavefronts-
+ * per-workgroup in {1, 2, 4, 8}, each staging mode that fits and one or two
+ * ants per wavefront, apply the fastest per bucket and report them (info:
+ * max_buckets entries).  Colonies must not be halted; pheromone and colony
+ * state are left unchanged. */
 int aco_hip_autotune(AcoHipContext *ctx, int launches, AcoHipBucketInfo *info, int max_buckets);
 
 /* ---- kernels (one launch each, asynchronous on the context stream) ------ */

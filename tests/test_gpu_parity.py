@@ -9,6 +9,7 @@ import pytest
 
 from vehicleroutingproblem_amd import HipSolver, Instance, Params
 from vehicleroutingproblem_amd import _native as N
+from vehicleroutingproblem_amd.solver import AcoError
 import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -84,7 +85,8 @@ def make_pair(name, **over):
     p = Params(**kw)
     s = HipSolver(inst, p, seeds=[seed])
     o = O.Oracle(inst, cand_k=p.cand_k, ants=p.ants, alpha=p.alpha, beta=p.beta, rho=p.rho,
-                 tau0=p.tau0, Q=p.Q, depot_w=p.depot_close_weight, knn_key=p.knn_key, seed=seed)
+                 tau0=p.tau0, Q=p.Q, depot_w=p.depot_close_weight, knn_key=p.knn_key, seed=seed,
+                 dist_mode=p.dist_mode)
     return inst, s, o
 
 
@@ -156,6 +158,57 @@ def test_construct_matches_oracle(gpu, name, stage, fm):
         assert len(np.unique(cust)) == len(cust) == inst.n - gu[a]
         assert o.route_cost(t) == gc[a]
     s.close()
+
+
+# two ants per wavefront (k_construct_pair) takes candidate lists of up to 32
+PAIR_CASES = [n for n, (_, kw) in CASES.items() if kw["cand_k"] <= 32]
+
+
+@pytest.mark.parametrize("name", PAIR_CASES)
+@pytest.mark.parametrize("stage,fm,dist", [(1, 1, 0), (0, 1, 0), (0, 0, 0), (0, 1, 1)])
+def test_pair_layout_matches_oracle(gpu, name, stage, fm, dist):
+    """Two ants per wavefront: four epochs give the oracle's tours, costs,
+    unvisited counts, colony state and pheromone, with
+    rows staged in LDS or read from global memory, with and without the
+    all-pairs fallback weights, in both distance modes."""
+    inst, s, o = make_pair(name, fallback_matrix=fm, dist_mode=dist)
+    try:
+        s.set_bucket_shape(0, 4, stage, ants_per_wave=2)
+    except AcoError:
+        pytest.skip("staged rows do not fit the LDS with two ants per wavefront")
+    assert s.buckets()[0]["ants_per_wave"] == 2
+    assert s.stage_lds == stage
+    s.prepare()
+    for e in range(4):
+        s.epochs(1)
+        o.epochs(1)
+        _compare_tours(s, o)
+        gs, os_ = s.colony_state(), o.state()
+        for key in ("epoch", "best_ant", "best_len", "feasible_ants", "global_best_cost"):
+            assert gs[key] == os_[key], (e, key, gs[key], os_[key])
+    tg, cg = s.tau()
+    to, co = o.tau()
+    assert np.array_equal(tg, to) and np.array_equal(cg, co)
+    s.close()
+
+
+@pytest.mark.parametrize("ants,apg", [(33, 4), (7, 1), (2, 3), (1, 2)])
+def test_pair_layout_ragged_ant_counts(gpu, ants, apg):
+    """Odd ant counts leave the last wavefront with one ant; workgroups past
+    the colony's ants exit; the tours still equal the one-ant layout's."""
+    mk, kw = CASES["c1_n100"]
+    inst = mk()
+    p = Params(**dict(kw, ants=ants))
+    out = []
+    for apw in (1, 2):
+        with HipSolver(inst, p, seeds=[99]) as s:
+            s.set_bucket_shape(0, apg, -1, ants_per_wave=apw)
+            assert s.buckets()[0]["ants_per_wave"] == apw
+            s.prepare()
+            s.epochs(4)
+            out.append(s.tours())
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("name,fm", [("c1_n100", 1), ("n300_clustered", 1), ("smoke_unit", 1),

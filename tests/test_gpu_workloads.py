@@ -13,6 +13,8 @@ compared with the CPU oracle bit for bit.
   all 512 colonies checked for route structure, capacity and exact cost.
 The reference order being restated is src/cuda/aco_cuda.cu:331-429.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -114,8 +116,11 @@ def test_c2_headline_shape_matches_oracle(gpu):
     inst = W.c2_instance(1)
     p = W.c2_params()
     with HipSolver(inst, p, seeds=[1]) as s:
-        # the shape the benchmark runs: LDS-staged rows, 4 ants per workgroup
-        assert s.ants_per_group == 4 and s.stage_lds == 1
+        # the shape the benchmark runs under the automatic layout policy:
+        # LDS-staged rows, 4 ants per workgroup, one ant per wavefront
+        if os.environ.get("ACO_HIP_PAIR", "-1") == "-1":
+            assert s.ants_per_group == 4 and s.stage_lds == 1
+            assert s.buckets()[0]["ants_per_wave"] == 1
         o = oracle_for(inst, p, 1)
         s.prepare()
         s.run(fixed_epochs=1, chunk_epochs=1, use_graph=True)

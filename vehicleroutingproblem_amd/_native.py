@@ -43,7 +43,7 @@ class AcoHipInstance(C.Structure):
 class AcoHipBucketInfo(C.Structure):
     _fields_ = [("n1_min", C.c_int), ("n1_max", C.c_int), ("instances", C.c_int),
                 ("colonies", C.c_int), ("ants_per_group", C.c_int), ("stage_lds", C.c_int),
-                ("lds_bytes", C.c_int), ("tuned_ms", C.c_float)]
+                ("lds_bytes", C.c_int), ("tuned_ms", C.c_float), ("ants_per_wave", C.c_int)]
 
 
 class AcoHipParams(C.Structure):
@@ -149,6 +149,7 @@ HIP_SYMBOLS = {
     "aco_hip_num_buckets": (C.c_int, [ctx_p]),
     "aco_hip_get_bucket": (C.c_int, [ctx_p, C.c_int, C.POINTER(AcoHipBucketInfo)]),
     "aco_hip_set_bucket_shape": (C.c_int, [ctx_p, C.c_int, C.c_int, C.c_int]),
+    "aco_hip_set_bucket_layout": (C.c_int, [ctx_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "aco_hip_autotune": (C.c_int, [ctx_p, C.c_int, C.POINTER(AcoHipBucketInfo), C.c_int]),
     "aco_hip_prior_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_uint32]),
     "aco_hip_prior_destroy": (None, [C.c_void_p]),

@@ -1,0 +1,25 @@
+/*
+ * aco_construct_tab.hip - construction kernels whose fallback roulette
+ * reads its weights from the pheromone row and the taupow / eta tables.
+ */
+#include "aco_construct.h"
+
+#define ACO_CONSTRUCT_VARIANTS(R)                                                          \
+  template __global__ void k_construct<true, false, R, false, false>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct<false, false, R, false, false>(DevCtx, int, int, const int *); \
+  template __global__ void k_construct<true, true, R, false, false>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct<false, true, R, false, false>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct<true, false, R, true, false>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct<false, false, R, true, false>(DevCtx, int, int, const int *);
+ACO_CONSTRUCT_VARIANTS(1)
+ACO_CONSTRUCT_VARIANTS(2)
+ACO_CONSTRUCT_VARIANTS(3)
+ACO_CONSTRUCT_VARIANTS(4)
+#undef ACO_CONSTRUCT_VARIANTS
+
+template __global__ void k_construct_pair<true, false, false, false>(DevCtx, int, int, const int *);
+template __global__ void k_construct_pair<false, false, false, false>(DevCtx, int, int, const int *);
+template __global__ void k_construct_pair<true, true, false, false>(DevCtx, int, int, const int *);
+template __global__ void k_construct_pair<false, true, false, false>(DevCtx, int, int, const int *);
+template __global__ void k_construct_pair<true, false, true, false>(DevCtx, int, int, const int *);
+template __global__ void k_construct_pair<false, false, true, false>(DevCtx, int, int, const int *);

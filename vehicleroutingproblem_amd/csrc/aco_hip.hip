@@ -93,12 +93,16 @@ static void free_pools(AcoHipContext *c) {
 static int fs_stride_of(int n1) { return (n1 + 15) & ~15; }
 
 /* LDS bytes of one construct workgroup for an instance of n1 nodes (layout
- * of k_construct): staged candidate rows (+64 padding entries), depot scores
- * and coordinates; the computed-fallback tables (taupow, and eta unless
- * eta_len == 0) when the weight matrix is off; per wave a state array of S
- * u16, a tour buffer of `stride` u16 and a 128-byte store sink. */
-static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on, int stride) {
-  size_t b = (size_t)W * ((size_t)fs_stride_of(n1) * 2 + r16((size_t)stride * 2) + 128);
+ * of k_construct and k_construct_pair): staged candidate rows (+64 padding
+ * entries), depot scores and coordinates; the computed-fallback tables
+ * (taupow, and eta unless eta_len == 0) when the weight matrix is off; per
+ * wave a state array of S u16, a tour buffer of `stride` u16 and a 128-byte
+ * store sink, or with `pair` two state arrays (the pair writes its tours to
+ * global memory). */
+static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on, int stride,
+                      int pair) {
+  const size_t state = (size_t)fs_stride_of(n1) * 2;
+  size_t b = (size_t)W * (pair ? 2 * state : state + r16((size_t)stride * 2) + 128);
   if (!fs_on) b += 256 * 4 + r16((size_t)eta_len * 4);
   if (stage) {
     const size_t ent = (size_t)n1 * k + 64;
@@ -216,22 +220,36 @@ static int env_int(const char *name, int dflt) {
   return (s && *s) ? atoi(s) : dflt;
 }
 
-/* Ants per workgroup and LDS staging of a bucket.  Preference: the
- * requested ants per workgroup (W_req, clamped to 1..8 and the ant count),
- * staged rows with the eta table, staged rows without it, then (unless
- * staging was forced) global rows, then fewer ants per workgroup. */
-static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_stage) {
-  int W = std::max(1, std::min(8, std::min(W_req, c->prm.ants)));
+/* Ants per workgroup, LDS staging and ants per wavefront of a bucket.
+ * Staging preference: the requested wavefronts per workgroup (W_req,
+ * clamped to 1..8 and the ant count), staged rows with the eta table, staged
+ * rows without it, then (unless staging was forced) global rows, then fewer
+ * wavefronts per workgroup.  want_pair: 1 two ants per wavefront when the
+ * candidate lists allow it (k <= 32), 0 one, -1 automatic: two when the
+ * launch has more ants than the device has SIMDs (the wavefronts then share
+ * SIMDs and the launch is bound by the instructions issued per ant, which a
+ * pair halves); one otherwise (each ant's chain of decisions is the bound). */
+static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_stage,
+                        int want_pair = -1) {
+  int kmax = 1;
+  for (int i : lb.insts) kmax = std::max(kmax, c->inst[i].k);
+  int pair = 0;
+  if (kmax <= 32) {
+    if (want_pair < 0) want_pair = c->pair_mode;
+    if (want_pair > 0) pair = 1;
+    else if (want_pair < 0)
+      pair = (long long)lb.cols.size() * c->prm.ants > 4LL * c->num_cu ? 1 : 0;
+  }
+  const int apw = pair ? 2 : 1;
+  int W = std::max(1, std::min(8, std::min(W_req, (c->prm.ants + apw - 1) / apw)));
   int stage = want_stage == 0 ? 0 : 1;
   int eta_in_lds = 1;
   size_t need = 0;
-  int kmax = 1;
-  for (int i : lb.insts) kmax = std::max(kmax, c->inst[i].k);
   auto recompute = [&]() {
     need = 0;
     for (int i : lb.insts)
       need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0, W,
-                                    stage, c->fs_on, c->max_stride));
+                                    stage, c->fs_on, c->max_stride, pair));
   };
   recompute();
   if (need > LDS_LIMIT) {
@@ -245,7 +263,7 @@ static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_
    * latency (n=1000, 32 colonies of 256 ants: 1.09 ms per epoch against
    * 2.47 ms staged) */
   if (want_stage < 0 && need <= LDS_LIMIT) {
-    const long long groups = (long long)lb.cols.size() * ((c->prm.ants + W - 1) / W);
+    const long long groups = (long long)lb.cols.size() * ((c->prm.ants + W * apw - 1) / (W * apw));
     const long long per_round = (long long)c->num_cu * (long long)(LDS_LIMIT / need);
     if (groups > per_round) need = LDS_LIMIT + 1; /* take the global-row branch below */
   }
@@ -270,7 +288,8 @@ static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_
   lb.stage = stage;
   lb.eta_lds = eta_in_lds;
   lb.rows = (kmax + 15) / 16;
-  lb.groups = (c->prm.ants + W - 1) / W;
+  lb.pair = pair;
+  lb.groups = (c->prm.ants + W * apw - 1) / (W * apw);
   lb.lds = need;
   return ACO_HIP_OK;
 }
@@ -467,6 +486,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       off += (int)lb.cols.size();
     }
   }
+  c->pair_mode = env_int("ACO_HIP_PAIR", -1);
   const int W0 = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
   int want_stage = p->stage_lds;
   if (want_stage < 0) want_stage = env_int("ACO_HIP_STAGE_LDS", -1);
@@ -646,14 +666,26 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.stats = nullptr;
   P.done = c->d_done;
 
-#define ACO_VARIANTS(R)                                                                         \
-  (const void *)k_construct<true, false, R, false>,                                            \
-      (const void *)k_construct<false, false, R, false>,                                       \
-      (const void *)k_construct<true, true, R, false>,                                         \
-      (const void *)k_construct<false, true, R, false>,                                        \
-      (const void *)k_construct<true, false, R, true>, (const void *)k_construct<false, false, R, true>
-  const void *variants[] = {ACO_VARIANTS(1), ACO_VARIANTS(2), ACO_VARIANTS(3), ACO_VARIANTS(4)};
+#define ACO_VARIANTS(R, F)                                                                      \
+  (const void *)k_construct<true, false, R, false, F>,                                         \
+      (const void *)k_construct<false, false, R, false, F>,                                    \
+      (const void *)k_construct<true, true, R, false, F>,                                      \
+      (const void *)k_construct<false, true, R, false, F>,                                     \
+      (const void *)k_construct<true, false, R, true, F>,                                      \
+      (const void *)k_construct<false, false, R, true, F>
+#define ACO_PAIR_VARIANTS(F)                                                                    \
+  (const void *)k_construct_pair<true, false, false, F>,                                       \
+      (const void *)k_construct_pair<false, false, false, F>,                                  \
+      (const void *)k_construct_pair<true, true, false, F>,                                    \
+      (const void *)k_construct_pair<false, true, false, F>,                                   \
+      (const void *)k_construct_pair<true, false, true, F>,                                    \
+      (const void *)k_construct_pair<false, false, true, F>
+  const void *variants[] = {ACO_VARIANTS(1, true),  ACO_VARIANTS(2, true),  ACO_VARIANTS(3, true),
+                            ACO_VARIANTS(4, true),  ACO_VARIANTS(1, false), ACO_VARIANTS(2, false),
+                            ACO_VARIANTS(3, false), ACO_VARIANTS(4, false), ACO_PAIR_VARIANTS(true),
+                            ACO_PAIR_VARIANTS(false)};
 #undef ACO_VARIANTS
+#undef ACO_PAIR_VARIANTS
   for (const void *f : variants)
     HIPC(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
   HIPC(c, hipFuncSetAttribute((const void *)k_select_deposit,
@@ -680,6 +712,7 @@ static void bucket_info(const LaunchBucket &lb, AcoHipBucketInfo *info) {
   info->stage_lds = lb.stage;
   info->lds_bytes = (int)lb.lds;
   info->tuned_ms = lb.tuned_ms;
+  info->ants_per_wave = lb.pair ? 2 : 1;
 }
 
 extern "C" int aco_hip_get_bucket(const AcoHipContext *c, int b, AcoHipBucketInfo *info) {
@@ -705,11 +738,16 @@ static void sync_shape0(AcoHipContext *c) {
 }
 
 extern "C" int aco_hip_set_bucket_shape(AcoHipContext *c, int b, int apg, int stage) {
+  return aco_hip_set_bucket_layout(c, b, apg, stage, 0);
+}
+
+extern "C" int aco_hip_set_bucket_layout(AcoHipContext *c, int b, int apg, int stage, int apw) {
   if (!c || !c->loaded) return ACO_HIP_ERR_INVALID_INPUT;
-  if (b < 0 || b >= (int)c->buckets.size() || apg < 1 || apg > 8 || stage < -1 || stage > 1)
+  if (b < 0 || b >= (int)c->buckets.size() || apg < 1 || apg > 8 || stage < -1 || stage > 1 ||
+      apw < 0 || apw > 2)
     return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "bad bucket shape request");
   LaunchBucket trial = c->buckets[b];
-  int rc = choose_shape(c, trial, apg, stage);
+  int rc = choose_shape(c, trial, apg, stage, apw == 0 ? -1 : apw - 1);
   if (rc) return rc;
   trial.tuned_ms = 0.0f;
   c->buckets[b] = trial;
@@ -771,43 +809,63 @@ extern "C" int aco_hip_init_tau(AcoHipContext *c) {
   return ACO_HIP_OK;
 }
 
-template <int R>
-static void launch_construct_rows(AcoHipContext *c, const LaunchBucket &lb, hipStream_t st,
-                                  bool stats) {
+/* One launch of the construction variant that matches the bucket's shape:
+ * ANTS = 1 (k_construct, ROWS = R) or 2 (k_construct_pair), FSM = fallback
+ * weights from the all-pairs weight matrix.  Launches of several colonies
+ * keep each colony's workgroups on one XCD (the counter variants keep the
+ * plain order). */
+template <int ANTS, int R, bool FSM>
+static void launch_variant(const AcoHipContext *c, const LaunchBucket &lb, hipStream_t st,
+                           bool stats) {
   DevCtx P = c->P;
   P.eta_lds = lb.eta_lds;
   const int *cols = c->d_colmap + lb.col_off;
   dim3 grid((unsigned)(lb.cols.size() * lb.groups)), block((unsigned)(64 * lb.W));
-  /* launches of several colonies keep each colony's workgroups on one XCD
-   * (the counter variants keep the plain order) */
   const bool xcd = lb.cols.size() > 1;
+#define ACO_GO(ST, SS, X)                                                                       \
+  do {                                                                                          \
+    if constexpr (ANTS == 2)                                                                    \
+      hipLaunchKernelGGL((k_construct_pair<ST, SS, X, FSM>), grid, block, lb.lds, st, P,        \
+                         lb.groups, lb.W, cols);                                                \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_construct<ST, SS, R, X, FSM>), grid, block, lb.lds, st, P,          \
+                         lb.groups, lb.W, cols);                                                \
+  } while (0)
   if (lb.stage && !stats && xcd)
-    hipLaunchKernelGGL((k_construct<true, false, R, true>), grid, block, lb.lds, st, P,
-                       lb.groups, lb.W, cols);
+    ACO_GO(true, false, true);
   else if (!stats && xcd)
-    hipLaunchKernelGGL((k_construct<false, false, R, true>), grid, block, lb.lds, st, P,
-                       lb.groups, lb.W, cols);
+    ACO_GO(false, false, true);
   else if (lb.stage && !stats)
-    hipLaunchKernelGGL((k_construct<true, false, R, false>), grid, block, lb.lds, st, P,
-                       lb.groups, lb.W, cols);
+    ACO_GO(true, false, false);
   else if (!stats)
-    hipLaunchKernelGGL((k_construct<false, false, R, false>), grid, block, lb.lds, st, P,
-                       lb.groups, lb.W, cols);
+    ACO_GO(false, false, false);
   else if (lb.stage)
-    hipLaunchKernelGGL((k_construct<true, true, R, false>), grid, block, lb.lds, st, P,
-                       lb.groups, lb.W, cols);
+    ACO_GO(true, true, false);
   else
-    hipLaunchKernelGGL((k_construct<false, true, R, false>), grid, block, lb.lds, st, P,
-                       lb.groups, lb.W, cols);
+    ACO_GO(false, true, false);
+#undef ACO_GO
+}
+
+template <bool FSM>
+static void launch_bucket_f(const AcoHipContext *c, const LaunchBucket &lb, hipStream_t st,
+                            bool stats) {
+  if (lb.pair) {
+    launch_variant<2, 1, FSM>(c, lb, st, stats);
+    return;
+  }
+  switch (lb.rows) {
+    case 1: launch_variant<1, 1, FSM>(c, lb, st, stats); break;
+    case 2: launch_variant<1, 2, FSM>(c, lb, st, stats); break;
+    case 3: launch_variant<1, 3, FSM>(c, lb, st, stats); break;
+    default: launch_variant<1, 4, FSM>(c, lb, st, stats); break;
+  }
 }
 
 static void launch_bucket(AcoHipContext *c, const LaunchBucket &lb, hipStream_t st, bool stats) {
-  switch (lb.rows) {
-    case 1: launch_construct_rows<1>(c, lb, st, stats); break;
-    case 2: launch_construct_rows<2>(c, lb, st, stats); break;
-    case 3: launch_construct_rows<3>(c, lb, st, stats); break;
-    default: launch_construct_rows<4>(c, lb, st, stats); break;
-  }
+  if (c->P.fscore)
+    launch_bucket_f<true>(c, lb, st, stats);
+  else
+    launch_bucket_f<false>(c, lb, st, stats);
 }
 
 /* One construct launch per size bucket: bucket 0 on the context stream, the
@@ -854,12 +912,14 @@ extern "C" int aco_hip_autotune(AcoHipContext *c, int launches, AcoHipBucketInfo
   for (size_t b = 0; b < c->buckets.size(); ++b) {
     LaunchBucket best = c->buckets[b];
     float best_ms = -1.0f;
+    for (int pair = 0; pair <= 1; ++pair)
     for (int stage = 1; stage >= 0; --stage)
       for (int W = 1; W <= 8; W <<= 1) {
-        if (W > c->prm.ants) continue;
+        if (W * (pair + 1) > c->prm.ants + pair) continue;
         LaunchBucket trial = c->buckets[b];
-        if (choose_shape(c, trial, W, stage) != ACO_HIP_OK) continue;
-        if (trial.W != W || trial.stage != stage) continue; /* does not fit as asked */
+        if (choose_shape(c, trial, W, stage, pair) != ACO_HIP_OK) continue;
+        /* does not fit as asked (or no pairs for lists above 32) */
+        if (trial.W != W || trial.stage != stage || trial.pair != pair) continue;
         launch_bucket(c, trial, c->stream, stats);          /* warm caches */
         HIPC(c, hipEventRecord(a, c->stream));
         for (int r = 0; r < launches; ++r) launch_bucket(c, trial, c->stream, stats);

@@ -174,11 +174,14 @@ static inline __host__ __device__ int aco_significant(double prev, double now, d
   return rel + eps >= min_rel;
 }
 
-/* kernels (aco_kernels.hip) */
+/* kernels (aco_kernels.hip; construction: aco_construct.h, instantiated in
+ * aco_construct_fsm.hip for FSM = true and aco_construct_tab.hip for false) */
 __global__ void k_knn(DevCtx P, int total_rows, int knn_key, float depot_w);
 __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
-template <bool STAGE, bool STATS, int ROWS, bool XCD>
+template <bool STAGE, bool STATS, int ROWS, bool XCD, bool FSM>
 __global__ void k_construct(DevCtx P, int groups_per_colony, int W, const int *cols);
+template <bool STAGE, bool STATS, bool XCD, bool FSM>
+__global__ void k_construct_pair(DevCtx P, int groups_per_colony, int W, const int *cols);
 __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
 __global__ void k_evaporate_flat(DevCtx P, const long long *item_off, const long long *slot_off,
                                  int ncol, int eta_lds, const int *cmap);

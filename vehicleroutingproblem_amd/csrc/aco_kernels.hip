@@ -1,85 +1,18 @@
 /*
- * aco_kernels.hip - gfx950 kernels of the ACO-CVRP hot path.
+ * aco_kernels.hip - gfx950 kernels of the ACO-CVRP hot path around tour
+ * construction (aco_construct.h): kNN candidate lists, pheromone fill,
+ * evaporation, best selection with the stop rules, deposits and route costs.
  *
- * One wavefront (64 lanes) builds one ant's tour.  Per decision the lanes
- * hold one candidate each: a wavefront scan of the candidate weights, one
- * 64-bit ballot and one scalar bit-scan pick the next customer.  Candidate
- * records, candidate pheromone, heuristic tables and the per-ant
- * visited/demand state live in LDS.  All draws come from Philox4x32-10, all
- * transcendental math from host-built tables, so tours replay bit for bit
- * from (seed, epoch, ant) and equal the CPU oracle's.
- *
- * Compiled with -ffp-contract=off: every float expression is evaluated as
- * written, in the order the oracle restates.
+ * All draws come from Philox4x32-10, all transcendental math from host-built
+ * tables, so tours replay bit for bit from (seed, epoch, ant) and equal the
+ * CPU oracle's.  Compiled with -ffp-contract=off: every float expression is
+ * evaluated as written, in the order the oracle restates.
  */
 #include <hip/hip_runtime.h>
 
-#include "aco_kernels.h"
 #include "aco_inline.h"
-
-#define VISITED 0xFFFFu
-#define FB_V 16
-#define FB_CHUNK 1024
-
-/* ---- wavefront primitives ------------------------------------------------ */
-
-__device__ __forceinline__ float f_dpp_row_shr1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float f_dpp_row_shr2(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float f_dpp_row_shr4(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float f_dpp_row_shr8(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, true));
-}
-
-__device__ __forceinline__ float f_readlane(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-/* Inclusive scan: Kogge-Stone by row_shr 1,2,4,8 inside each 16-lane row
- * (lanes shifted in from outside the row read 0), then row r adds the last
- * prefix of row r-1 for r = 1..rows-1.  The CPU oracle restates this order. */
-__device__ __forceinline__ float wave_scan_rows(float v, int lane, int rows) {
-  v = v + f_dpp_row_shr1(v);
-  v = v + f_dpp_row_shr2(v);
-  v = v + f_dpp_row_shr4(v);
-  v = v + f_dpp_row_shr8(v);
-  if (rows > 1) {
-    const float c = f_readlane(v, 15);
-    if ((lane >> 4) == 1) v = v + c;
-  }
-  if (rows > 2) {
-    const float c = f_readlane(v, 31);
-    if ((lane >> 4) == 2) v = v + c;
-  }
-  if (rows > 3) {
-    const float c = f_readlane(v, 47);
-    if ((lane >> 4) == 3) v = v + c;
-  }
-  return v;
-}
-
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    uint64_t o = __shfl_xor(v, off, 64);
-    v = o < v ? o : v;
-  }
-  return v;
-}
-
-__device__ __forceinline__ long long wave_sum_i64(long long v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-__device__ __forceinline__ uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
-
-__device__ __forceinline__ int first_lane(uint64_t m) { return __builtin_ctzll(m); }
-__device__ __forceinline__ int last_lane(uint64_t m) { return 63 - __builtin_clzll(m); }
+#include "aco_kernels.h"
+#include "aco_wave.h"
 
 /* ---- kNN candidate lists ------------------------------------------------- */
 
@@ -169,445 +102,6 @@ __global__ void __launch_bounds__(256) k_fill_tau(uint8_t *pher, size_t bytes, u
        e += (size_t)gridDim.x * blockDim.x)
     reinterpret_cast<uint4 *>(pher)[e] = v;
 }
-
-/* ---- tour construction --------------------------------------------------- */
-
-/* Copies `bytes` (multiple of 16, 16-byte aligned on both sides) with the
- * whole workgroup, eight loads in flight per thread. */
-__device__ __forceinline__ void wg_copy16(void *dst, const void *src, size_t bytes) {
-  const int nvec = (int)(bytes / 16);
-  const int nt = (int)blockDim.x;
-  uint4 *d = reinterpret_cast<uint4 *>(dst);
-  const uint4 *s = reinterpret_cast<const uint4 *>(src);
-  int e = (int)threadIdx.x;
-  for (; e + 7 * nt < nvec; e += 8 * nt) {
-    uint4 v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = s[e + u * nt];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) d[e + u * nt] = v[u];
-  }
-  for (; e < nvec; e += nt) d[e] = s[e];
-}
-
-struct FbOut {
-  int move;
-  int mdem;
-  float T;
-};
-
-/* The fallback roulette partitions the nodes 0..n into chunks of 1024:
- * node j = c*1024 + l*16 + i belongs to chunk c, lane l, slot i (the depot is
- * visited from the start, so it never carries weight).  A lane's 16 nodes are
- * contiguous, and so are its fallback weights and states. */
-
-/* Weights of the 16 nodes b..b+15 of one lane from the all-pairs weight row
- * (0 when visited, too heavy, or past the last node).  Lanes past the last
- * chunk read node 0's group and keep nothing. */
-__device__ __forceinline__ float fb_lane_fs(float (&w)[FB_V], int b0, int n1, int room,
-                                            const float *__restrict__ frow,
-                                            const uint16_t *state) {
-  const bool live = b0 < n1;
-  const int b = live ? b0 : 0;
-  const float4 *f4 = reinterpret_cast<const float4 *>(frow + b);
-  const uint4 *s4 = reinterpret_cast<const uint4 *>(state + b);
-  const float4 fa = f4[0], fb = f4[1], fc = f4[2], fd = f4[3];
-  const uint4 sa = s4[0], sb = s4[1];
-  const float fv[FB_V] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w,
-                          fc.x, fc.y, fc.z, fc.w, fd.x, fd.y, fd.z, fd.w};
-  const uint32_t sw[8] = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y, sb.z, sb.w};
-  const int lim = live ? room : -1;
-  float s = 0.0f;
-#pragma unroll
-  for (int i = 0; i < FB_V; ++i) {
-    const int st = (int)((sw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
-    w[i] = st <= lim ? fv[i] : 0.0f;
-    s = s + w[i];
-  }
-  return s;
-}
-
-/* Same weights computed from the pheromone row, the coordinates and the
- * taupow / eta tables (used when the all-pairs weight matrix is off). */
-__device__ __forceinline__ float fb_lane_tables(float (&w)[FB_V], int b0, int n1, int room,
-                                                int2 pc, const int2 *xy, const float *taupow,
-                                                const float *eta, const uint16_t *state,
-                                                const uint8_t *__restrict__ trow, bool real) {
-  uint32_t st[FB_V], tq[FB_V];
-  int2 pj[FB_V];
-#pragma unroll
-  for (int i = 0; i < FB_V; ++i) {
-    const int j = b0 + i;
-    const int jc = j < n1 ? j : 0;
-    st[i] = state[jc];
-    tq[i] = trow[jc];
-    pj[i] = xy[jc];
-  }
-  float s = 0.0f;
-#pragma unroll
-  for (int i = 0; i < FB_V; ++i) {
-    const uint32_t d = real ? edge_code_t<true>(pc, pj[i]) : edge_code_t<false>(pc, pj[i]);
-    const float v = taupow[tq[i]] * eta[d];
-    w[i] = (b0 + i < n1 && (int)st[i] <= room) ? v : 0.0f;
-    s = s + w[i];
-  }
-  return s;
-}
-
-/* Fallback roulette over every unvisited customer that fits (the exact
- * two-pass restatement of select_global_fallback, aco_cuda_kernels.cu:336-430):
- * chunk totals, then the chunk, the lane and the slot holding the threshold.
- * frow != NULL selects the all-pairs weight row; otherwise the weights are
- * computed from trow and the tables.  Returns the node (or -1), its demand and
- * the total weight. */
-__device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow, const int2 *xy,
-                                              const float *taupow, const float *eta,
-                                              const uint16_t *state,
-                                              const uint8_t *__restrict__ trow, int n1, int cur,
-                                              int room, float u, int lane, bool real) {
-  const int nch = (n1 + FB_CHUNK - 1) / FB_CHUNK;
-  const int2 pc = xy[cur];
-  float w[FB_V];
-  float Rv = 0.0f, Cv = 0.0f, R = 0.0f, s = 0.0f, Pl = 0.0f;
-  for (int c = 0; c < nch; ++c) {
-    const int b0 = c * FB_CHUNK + lane * FB_V;
-    s = frow ? fb_lane_fs(w, b0, n1, room, frow, state)
-             : fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow, real);
-    Pl = wave_scan_rows(s, lane, 4);
-    const float Cc = f_readlane(Pl, 63);
-    R = R + Cc;
-    if (lane == c) {
-      Rv = R;
-      Cv = Cc;
-    }
-  }
-  FbOut out;
-  out.T = R;
-  out.move = -1;
-  out.mdem = 0;
-  if (!(R > 0.0f)) return out;
-  const float thr = u * R;
-  const uint64_t m = ballot64(lane < nch && Rv > thr);
-  const int cs = m ? first_lane(m) : last_lane(ballot64(lane < nch && Cv > 0.0f));
-  const float base = cs > 0 ? f_readlane(Rv, cs - 1) : 0.0f;
-  const float thr_c = thr - base;
-  if (cs != nch - 1) {
-    const int b0 = cs * FB_CHUNK + lane * FB_V;
-    s = frow ? fb_lane_fs(w, b0, n1, room, frow, state)
-             : fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow, real);
-    Pl = wave_scan_rows(s, lane, 4);
-  }
-  const uint64_t lm = ballot64(s > 0.0f && Pl > thr_c);
-  const int ls = lm ? first_lane(lm) : last_lane(ballot64(s > 0.0f));
-  const float pprev = ls > 0 ? f_readlane(Pl, ls - 1) : 0.0f;
-  const float thr_l = thr_c - pprev;
-  int is = -1, il = -1;
-  float p = 0.0f;
-#pragma unroll
-  for (int i = 0; i < FB_V; ++i) {
-    p = p + w[i];
-    if (w[i] > 0.0f) {
-      il = i;
-      if (is < 0 && p > thr_l) is = i;
-    }
-  }
-  if (is < 0) is = il;
-  const int isel = __builtin_amdgcn_readlane(is, ls);
-  out.move = cs * FB_CHUNK + ls * FB_V + isel;
-  out.mdem = __builtin_amdgcn_readfirstlane((int)state[out.move]);
-  return out;
-}
-
-/* Inclusive scan of the candidate weights with the row order of
- * wave_scan_rows, specialised for ROWS rows: row 1 takes lane 15 through a
- * row_bcast:15 DPP add (the same single addition as a readlane carry). */
-template <int ROWS>
-__device__ __forceinline__ float cand_scan(float v, int lane) {
-  v = v + f_dpp_row_shr1(v);
-  v = v + f_dpp_row_shr2(v);
-  v = v + f_dpp_row_shr4(v);
-  v = v + f_dpp_row_shr8(v);
-  if constexpr (ROWS >= 2)
-    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0x2, 0xF,
-                                                       false));
-  if constexpr (ROWS >= 3) {
-    const float c = f_readlane(v, 31);
-    if ((lane >> 4) == 2) v = v + c;
-  }
-  if constexpr (ROWS >= 4) {
-    const float c = f_readlane(v, 47);
-    if ((lane >> 4) == 3) v = v + c;
-  }
-  return v;
-}
-
-/* One wavefront per ant.  Per decision every lane reads entry `lane` of the
- * current node's candidate row (lanes >= k read past the row, in bounds by
- * the 64-entry padding, and are masked out of the ballots), then that node's
- * state (its demand, or VISITED); a DPP scan, two ballots and one bit-scan
- * select the move.  All control flow is uniform (scalar).  ROWS = 16-lane rows
- * the widest candidate list spans. */
-template <bool STAGE, bool STATS, int ROWS, bool XCD>
-__global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colony, int W,
-                                                   const int *cols) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  const unsigned long long t_kernel = STATS ? clock64() : 0ull;
-  /* cols: the colonies of this launch (one launch per size bucket) */
-  /* Workgroups are dispatched to the 8 XCDs round-robin (workgroup i to
-   * XCD i mod 8), and each XCD has its own L2.  With XCD set (launches of
-   * many colonies) they are renumbered so that XCD x runs the x-th
-   * contiguous run of logical workgroups: the ants of a colony (consecutive
-   * logical workgroups) stay on one XCD, whose L2 then holds the candidate
-   * rows of an eighth of the colonies instead of all of them. */
-  unsigned lb = blockIdx.x;
-  if constexpr (XCD) {
-    const unsigned nxcd = 8;
-    const unsigned xq = gridDim.x / nxcd, xr = gridDim.x % nxcd;
-    const unsigned xid = blockIdx.x % nxcd, xi = blockIdx.x / nxcd;
-    lb = xid * xq + (xid < xr ? xid : xr) + xi;
-  }
-  const int slot = lb / groups_per_colony;
-  const int group = lb - slot * groups_per_colony;
-  const int colony = cols ? cols[slot] : slot;
-  const ColonyDesc D = P.col[colony];
-  const AcoHipColonyState *S = P.state + colony;
-  if (S->halted) return; /* uniform over the workgroup */
-  const int epoch = S->epoch;
-  const InstDesc I = P.inst[D.inst];
-  const int n = I.n, k = I.k, K = I.K, Q = I.Q, n1 = n + 1, Sn = I.fs_stride;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const bool fs_on = P.fscore != nullptr;
-
-  /* ---- LDS carve-up and staging ---- */
-  const size_t ent = (size_t)n1 * k + 64; /* candidate entries incl. padding */
-  size_t off = 0;
-  const uint16_t *c_idx;
-  const float *c_score;
-  const float *c_dsc;
-  const int2 *c_xy;
-  if (STAGE) {
-    uint16_t *s_idx = reinterpret_cast<uint16_t *>(smem + off);
-    off += round16(ent * 2);
-    float *s_score = reinterpret_cast<float *>(smem + off);
-    off += round16(ent * 4);
-    float *s_dsc = reinterpret_cast<float *>(smem + off);
-    off += round16((size_t)n1 * 4);
-    int2 *s_xy = reinterpret_cast<int2 *>(smem + off);
-    off += round16((size_t)n1 * 8);
-    wg_copy16(s_idx, P.cidx + I.cand_off, round16(ent * 2));
-    wg_copy16(s_score, P.score + D.score_off, round16(ent * 4));
-    wg_copy16(s_dsc, P.dscore + I.node_off, round16((size_t)n1 * 4));
-    wg_copy16(s_xy, P.xy + I.node_off, round16((size_t)n1 * 8));
-    c_idx = s_idx;
-    c_score = s_score;
-    c_dsc = s_dsc;
-    c_xy = s_xy;
-  } else {
-    c_idx = P.cidx + I.cand_off;
-    c_score = P.score + D.score_off;
-    c_dsc = P.dscore + I.node_off;
-    c_xy = P.xy + I.node_off;
-  }
-  /* the tables of the computed fallback (only without the weight matrix);
-   * large coordinate ranges keep the eta table in global memory */
-  const float *c_taupow = P.taupow;
-  const float *c_eta = P.eta;
-  if (!fs_on) {
-    float *s_taupow = reinterpret_cast<float *>(smem + off);
-    off += 256 * 4;
-    wg_copy16(s_taupow, P.taupow, 256 * 4);
-    c_taupow = s_taupow;
-    if (P.eta_lds) {
-      float *s_eta = reinterpret_cast<float *>(smem + off);
-      off += round16((size_t)P.eta_len * 4);
-      wg_copy16(s_eta, P.eta, round16((size_t)P.eta_len * 4));
-      c_eta = s_eta;
-    }
-  }
-  uint16_t *s_state = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * Sn * 2);
-  off += (size_t)W * Sn * 2;
-  /* tour buffer of this wave: entry p of the tour (u16 node ids) */
-  uint16_t *s_tour = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * round16((size_t)D.stride * 2));
-  off += (size_t)W * round16((size_t)D.stride * 2);
-  /* 64 u16 per wave that lanes 1..63 write instead of the state and tour
-   * entries lane 0 writes, so those stores need no exec mask */
-  uint16_t *s_sink = reinterpret_cast<uint16_t *>(smem + off + (size_t)wave * 128);
-  __syncthreads();
-  if (STATS && threadIdx.x == 0)
-    atomicAdd(P.stats + (size_t)colony * ACO_STATS + 5, clock64() - t_kernel);
-
-  const int ant = group * W + wave;
-  if (ant >= D.m) return;
-
-  /* ---- reset (kernel_reset_ant_state, aco_cuda_kernels.cu:227-257): the
-   * state of node j is its demand while unvisited, VISITED after; the depot
-   * and the padding up to S start visited ---- */
-  const uint16_t *g_dem = P.dem + I.node_off;
-  for (int j = lane; j < Sn; j += 64)
-    s_state[j] = (j == 0 || j >= n1) ? (uint16_t)VISITED : g_dem[j];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-  const uint8_t *__restrict__ tau = P.pher + D.pher_off;
-  const float *__restrict__ fsm = fs_on ? P.fscore + D.fs_off : nullptr;
-  int32_t *tour_out = P.tours + D.tour_off + (size_t)ant * D.stride;
-  const uint32_t key0 = I.seed;
-  const uint32_t cid = (uint32_t)D.colony_in_inst;
-  const uint64_t kmask = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
-
-  const unsigned long long t_start = STATS ? clock64() : 0ull;
-  unsigned long long fallbacks = 0, fb_cycles = 0;
-  int rem_cnt = n;
-  int rem_dem = (int)I.total_demand;
-  uint32_t s = 0;
-  float rf0 = 0.0f, rf1 = 0.0f; /* lane l: uniform draws of decision (s & ~63) + l */
-  int pos = 0;
-  if (lane == 0) s_tour[0] = 0;
-  /* lane 0 stores to state[v] and tour[p + 1], the others to their sink */
-  const int wscale = lane == 0 ? 1 : 0;
-  uint16_t *const st_w = lane == 0 ? s_state : s_sink + lane;
-  uint16_t *const tr_w = lane == 0 ? s_tour + 1 : s_sink + lane;
-
-  for (int veh = 0; veh < K; ++veh) {
-    int cur = 0;
-    int room = Q; /* Q - load */
-    const int close_cap = (K - veh - 1) * Q; /* close allowed iff rem_dem <= close_cap */
-    /* the current node's row: lane t holds candidate t (index, score, state)
-     * and every lane the node's depot score.  After a move the next row is
-     * loaded at once, ahead of the close roulette and the bookkeeping: none
-     * of its states can change before its decision, since the only state
-     * written meanwhile is the move's own, which is not in its candidate
-     * list (lanes past k read other rows and are masked out). */
-    int j = (int)c_idx[lane];
-    float sc = c_score[lane];
-    float dsc_cur = c_dsc[0];
-    uint16_t st = s_state[j]; /* 16-bit loop variable: widened at its use */
-    /* aco_cuda_kernels.cu:492: the route goes on while rem_cnt > 0 and
-     * load < Q; cur = -1 marks a route closed by a roulette (one exit test) */
-    while (((rem_cnt - 1) | (room - 1) | cur) >= 0) {
-      const int sl = (int)(s & 63u);
-      if (sl == 0) {
-        uint32_t r[4];
-        aco_philox4x32_10(s + (uint32_t)lane, (uint32_t)ant, (uint32_t)epoch, cid, key0,
-                          ACO_PHILOX_KEY1, r);
-        rf0 = aco_u01(r[0]);
-        rf1 = aco_u01(r[1]);
-      }
-      ++s;
-      const float u0 = f_readlane(rf0, sl);
-
-      /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322) */
-      const float w = ((int)st <= room) ? sc : 0.0f;
-      const uint64_t emask = ballot64(w > 0.0f) & kmask;
-      const float Pf = cand_scan<ROWS>(w, lane);
-      int move, mdem;
-      float T;
-      if (__builtin_expect(emask != 0, 1)) {
-        /* T > 0 exactly when some weight is > 0 (sums of non-negatives) */
-        T = f_readlane(Pf, k - 1);
-        const float thr = u0 * T;
-        const uint64_t m = ballot64(Pf > thr) & emask;
-        const int sel = m ? first_lane(m) : last_lane(emask);
-        move = __builtin_amdgcn_readlane(j, sel);
-        mdem = __builtin_amdgcn_readlane((int)st, sel);
-      } else {
-        const unsigned long long f0 = STATS ? clock64() : 0ull;
-        const FbOut fb = fallback_select(fsm ? fsm + (size_t)cur * Sn : nullptr, c_xy, c_taupow,
-                                         c_eta, s_state, tau + (size_t)cur * Sn, n1, cur, room,
-                                         u0, lane, P.dist_real != 0);
-        move = fb.move;
-        mdem = fb.mdem;
-        T = fb.T;
-        if (STATS) {
-          fb_cycles += clock64() - f0;
-          ++fallbacks;
-        }
-      }
-      /* prefetch of the next row (row 0 when nothing was found) */
-      const int mv = move > 0 ? move : 0;
-      const int e2 = mv * k + lane;
-      const int j2 = (int)c_idx[e2];
-      const float sc2 = c_score[e2];
-      const float dsc2 = c_dsc[mv];
-      /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand,
-       * without branches: dsc_cur is the same in every lane; depot scores
-       * are finite or 0 (never NaN), so dsc > 0 is a test of the
-       * sign-and-magnitude bits.  A fallback that found nothing (move -1)
-       * closes the route whatever this roulette draws. */
-      {
-        const float thr2 = f_readlane(rf1, sl) * (T + dsc_cur);
-        const uint64_t cm = ballot64(__float_as_int(dsc_cur) > 0) & ballot64(thr2 >= T);
-        if (rem_dem <= close_cap && cm != 0) move = 0;
-      }
-      /* the move's bookkeeping, also without branches: with no move (a close,
-       * or a fallback that found nothing) lane 0 marks the depot visited
-       * again and writes tour[pos + 1] = 0, which the route close writes
-       * anyway; the loop then ends on cur = -1 */
-      const bool moved = move > 0;
-      const int mvw = moved ? move : 0;
-      const uint16_t st2 = s_state[j2]; /* read before state[move] changes: j2 != move */
-      st_w[mvw * wscale] = (uint16_t)VISITED;
-      tr_w[pos * wscale] = (uint16_t)mvw;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int md = moved ? mdem : 0;
-      pos += moved ? 1 : 0;
-      room -= md;
-      rem_dem -= md;
-      rem_cnt -= moved ? 1 : 0;
-      cur = moved ? move : -1;
-      j = j2;
-      sc = sc2;
-      dsc_cur = dsc2;
-      st = st2;
-    }
-    /* route close (aco_cuda_kernels.cu:547-552) */
-    ++pos;
-    if (lane == 0) s_tour[pos] = 0;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  /* write the tour out and sum its edge costs (int64 cost units, exact) */
-  long long cost = 0;
-  for (int p = lane; p <= pos; p += 64) {
-    const int v = s_tour[p];
-    tour_out[p] = v;
-    if (p >= 1) cost += edge_units(P, c_xy[s_tour[p - 1]], c_xy[v]);
-  }
-  cost = wave_sum_i64(cost);
-  if (lane == 0) {
-    const int a = D.ant_off + ant;
-    P.len[a] = pos + 1;
-    P.cost[a] = cost;
-    P.unvis[a] = rem_cnt;
-    if (rem_cnt == 0) P.feas[colony] = epoch + 1; /* some ant of this epoch is feasible */
-    if (STATS) {
-      unsigned long long *sc = P.stats + (size_t)colony * ACO_STATS;
-      atomicAdd(sc + 0, (unsigned long long)s);
-      atomicAdd(sc + 1, fallbacks);
-      atomicAdd(sc + 2, fb_cycles);
-      atomicAdd(sc + 3, clock64() - t_start);
-      atomicAdd(sc + 4, 1ull);
-    }
-  }
-}
-
-#define ACO_CONSTRUCT_VARIANTS(R)                                                          \
-  template __global__ void k_construct<true, false, R, false>(DevCtx, int, int, const int *);  \
-  template __global__ void k_construct<false, false, R, false>(DevCtx, int, int, const int *); \
-  template __global__ void k_construct<true, true, R, false>(DevCtx, int, int, const int *);   \
-  template __global__ void k_construct<false, true, R, false>(DevCtx, int, int, const int *);  \
-  template __global__ void k_construct<true, false, R, true>(DevCtx, int, int, const int *);   \
-  template __global__ void k_construct<false, false, R, true>(DevCtx, int, int, const int *);
-ACO_CONSTRUCT_VARIANTS(1)
-ACO_CONSTRUCT_VARIANTS(2)
-ACO_CONSTRUCT_VARIANTS(3)
-ACO_CONSTRUCT_VARIANTS(4)
-#undef ACO_CONSTRUCT_VARIANTS
 
 /* ---- evaporation ---------------------------------------------------------- */
 

@@ -173,6 +173,10 @@ class HipSolver:
             arr[i].cand_k = int(cand_k[i]) if cand_k is not None else 0
         self._check(self.lib.aco_hip_load(self._ctx, arr, len(self.instances),
                                           C.byref(self.params.to_c())))
+        self._read_shape()
+
+    def _read_shape(self) -> None:
+        """ants, ants_per_group and stage_lds of bucket 0 (the first instance's)."""
         k, m, stride, apg, stage = (C.c_int32() for _ in range(5))
         self._check(self.lib.aco_hip_get_shape(self._ctx, C.byref(k), C.byref(m),
                                                C.byref(stride), C.byref(apg), C.byref(stage)))
@@ -309,14 +313,18 @@ class HipSolver:
             out.append(self._bucket_dict(b))
         return out
 
-    def set_bucket_shape(self, bucket: int, ants_per_group: int, stage_lds: int = -1):
-        self._check(self.lib.aco_hip_set_bucket_shape(self._ctx, bucket, ants_per_group,
-                                                      stage_lds))
+    def set_bucket_shape(self, bucket: int, ants_per_group: int, stage_lds: int = -1,
+                         ants_per_wave: int = 0):
+        """ants_per_wave: 0 automatic, 1, or 2 (two ants per wavefront)."""
+        self._check(self.lib.aco_hip_set_bucket_layout(self._ctx, bucket, ants_per_group,
+                                                       stage_lds, ants_per_wave))
+        self._read_shape()
 
     def autotune(self, launches: int = 3) -> list:
         nb = self.lib.aco_hip_num_buckets(self._ctx)
         arr = (N.AcoHipBucketInfo * max(nb, 1))()
         self._check(self.lib.aco_hip_autotune(self._ctx, launches, arr, nb))
+        self._read_shape()
         return [self._bucket_dict(arr[i]) for i in range(nb)]
 
     def set_candidate_eta(self, eta, instance: int = 0):

@@ -64,7 +64,8 @@ def tune_shapes(solver: HipSolver, cache: str | None = DEFAULT_CACHE, launches: 
     for b, info in enumerate(buckets):
         hit = table.get(_shape_key(info, bucket_k(info), ants))
         if hit:
-            solver.set_bucket_shape(b, hit["ants_per_group"], hit["stage_lds"])
+            solver.set_bucket_shape(b, hit["ants_per_group"], hit["stage_lds"],
+                                    hit.get("ants_per_wave", 0))
         else:
             missing = True
     if missing:
@@ -72,6 +73,7 @@ def tune_shapes(solver: HipSolver, cache: str | None = DEFAULT_CACHE, launches: 
         for info in buckets:
             table[_shape_key(info, bucket_k(info), ants)] = {"ants_per_group": info["ants_per_group"],
                                                 "stage_lds": info["stage_lds"],
+                                                "ants_per_wave": info["ants_per_wave"],
                                                 "ms": info["tuned_ms"]}
         if cache:
             _store(cache, table)
