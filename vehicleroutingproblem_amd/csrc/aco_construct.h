@@ -182,6 +182,13 @@ __device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow,
   return out;
 }
 
+/* *(base + byte_off): a 32-bit byte offset from a wave-uniform base, so a
+ * global load takes the scalar-base form and an LDS load one address add. */
+template <class V>
+__device__ __forceinline__ V ld_at(const V *base, uint32_t byte_off) {
+  return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 /* Inclusive scan of the candidate weights with the row order of
  * wave_scan_rows, specialised for ROWS rows: row 1 takes lane 15 through a
  * row_bcast:15 DPP add (the same single addition as a readlane carry). */
@@ -191,9 +198,9 @@ __device__ __forceinline__ float cand_scan(float v, int lane) {
   v = v + f_dpp_row_shr2(v);
   v = v + f_dpp_row_shr4(v);
   v = v + f_dpp_row_shr8(v);
-  if constexpr (ROWS >= 2)
-    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0x2, 0xF,
-                                                       false));
+  if constexpr (ROWS >= 2) /* row 1: v = v[lane 15] + v, one DPP add (see pair_scan) */
+    asm("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0x2 bank_mask:0xf"
+        : "+v"(v));
   if constexpr (ROWS >= 3) {
     const float c = f_readlane(v, 31);
     if ((lane >> 4) == 2) v = v + c;
@@ -333,6 +340,9 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
   const int wscale = lane == 0 ? 1 : 0;
   uint16_t *const st_w = lane == 0 ? s_state : s_sink + lane;
   uint16_t *const tr_w = lane == 0 ? s_tour + 1 : s_sink + lane;
+  /* byte offsets of this lane's candidate entry in a row (index, score) */
+  uint32_t l2 = 2u * (uint32_t)lane, l4 = 4u * (uint32_t)lane;
+  asm("" : "+v"(l2), "+v"(l4)); /* keeps row offset + lane offset one add each */
 
   for (int veh = 0; veh < K; ++veh) {
     int cur = 0;
@@ -344,10 +354,10 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
      * of its states can change before its decision, since the only state
      * written meanwhile is the move's own, which is not in its candidate
      * list (lanes past k read other rows and are masked out). */
-    int j = (int)c_idx[lane];
+    uint32_t j = c_idx[lane];
     float sc = c_score[lane];
     float dsc_cur = c_dsc[0];
-    uint16_t st = s_state[j]; /* 16-bit loop variable: widened at its use */
+    uint32_t st = s_state[j];
     /* aco_cuda_kernels.cu:492: the route goes on while rem_cnt > 0 and
      * load < Q; cur = -1 marks a route closed by a roulette (one exit test) */
     while (((rem_cnt - 1) | (room - 1) | cur) >= 0) {
@@ -374,7 +384,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
         const float thr = u0 * T;
         const uint64_t m = ballot64(Pf > thr) & emask;
         const int sel = m ? first_lane(m) : last_lane(emask);
-        move = __builtin_amdgcn_readlane(j, sel);
+        move = __builtin_amdgcn_readlane((int)j, sel);
         mdem = __builtin_amdgcn_readlane((int)st, sel);
       } else {
         const unsigned long long f0 = STATS ? clock64() : 0ull;
@@ -389,11 +399,12 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
           ++fallbacks;
         }
       }
-      /* prefetch of the next row (row 0 when nothing was found) */
+      /* prefetch of the next row (row 0 when nothing was found), from the
+       * row's wave-uniform base */
       const int mv = move > 0 ? move : 0;
-      const int e2 = mv * k + lane;
-      const int j2 = (int)c_idx[e2];
-      const float sc2 = c_score[e2];
+      const uint32_t e2 = (uint32_t)(mv * k);
+      const uint32_t j2 = ld_at(c_idx, 2u * e2 + l2);
+      const float sc2 = ld_at(c_score, 4u * e2 + l4);
       const float dsc2 = c_dsc[mv];
       /* depot-close roulette (aco_cuda_kernels.cu:501-520) on remaining demand,
        * without branches: dsc_cur is the same in every lane; depot scores
@@ -411,7 +422,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
        * anyway; the loop then ends on cur = -1 */
       const bool moved = move > 0;
       const int mvw = moved ? move : 0;
-      const uint16_t st2 = s_state[j2]; /* read before state[move] changes: j2 != move */
+      const uint32_t st2 = s_state[j2]; /* read before state[move] changes: j2 != move */
       st_w[mvw * wscale] = (uint16_t)VISITED;
       tr_w[pos * wscale] = (uint16_t)mvw;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -462,13 +473,6 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
 
 
 /* ---- tour construction, two ants per wavefront ----------------------------- */
-
-/* *(base + byte_off): a 32-bit byte offset from a wave-uniform base, so a
- * global load takes the scalar-base form and an LDS load one address add. */
-template <class V>
-__device__ __forceinline__ V ld_at(const V *base, uint32_t byte_off) {
-  return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + byte_off);
-}
 
 /* The scan of cand_scan<2> in both 32-lane halves at once: row_shr 1, 2, 4,
  * 8 inside each 16-lane row, then rows 1 and 3 add lane 15 of rows 0 and 2
