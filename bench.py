@@ -34,13 +34,15 @@ HBM_PEAK_GBS = 8000.0
 # (fetch x2.0, write x1.0); the commands and figures are in
 # profiles/k_construct_n1000_m256_hbm_traffic.md (1 seed) and
 # profiles/k_construct_n1000_32seeds_hbm_traffic.md (32 seeds).
-CONSTRUCT_TRAFFIC_BYTES = {1: 20869168, 32: 310561551}
-# Instructions k_construct issues per decision at this workload (VALU + SALU +
-# LDS + branch SQ counters over decisions; profiles/k_construct_n1000_m256_sq_counters.md
-# for one seed, profiles/k_construct_n1000_32seeds_sq_counters.md for 32): a
+CONSTRUCT_TRAFFIC_BYTES = {1: 22648440, 32: 680404145}
+# Instructions the construct launch issues per ant-decision at this workload
+# (VALU + SALU + LDS + branch SQ counters over decisions;
+# profiles/k_construct_n1000_m256_sq_counters.md for one seed, one ant per
+# wavefront; profiles/k_construct_n1000_32seeds_sq_counters.md for 32 seeds,
+# two ants per wavefront, whose wavefront makes two decisions per pass): a
 # lone wavefront issues at most one instruction per 4-cycle slot of its SIMD,
 # so 4 x this is the issue bound of one decision in shader cycles.
-CONSTRUCT_INSTS_PER_DECISION = {1: 103.0, 32: 102.1}
+CONSTRUCT_INSTS_PER_DECISION = {1: 102.0, 32: 64.7}
 
 
 def tour_bytes(n, K, k):
@@ -470,7 +472,7 @@ def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int =
     states = [s.colony_state(i) for i in range(len(seeds))]
     epochs = [st["epoch"] - warmup for st in states]
     feasible_last = sum(float((s.tours(i)[3] == 0).mean()) for i in range(len(seeds))) / len(seeds)
-    shape = (s.ants_per_group, s.stage_lds)
+    shape = (s.ants_per_group, s.stage_lds, s.buckets()[0]["ants_per_wave"])
     s.close()
 
     def replay(timing: bool):
@@ -509,8 +511,9 @@ def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int =
         "epochs_timed": max(epochs),
         "feasible_ant_fraction_last_epoch": round(feasible_last, 4),
         "best_cost": [st["global_best_cost"] for st in states][:8],
-        "ants_per_group": shape[0], "stage_lds": shape[1],
-        "roofline": {"bound": "hbm", "kernel": "k_construct", "achieved": round(achieved, 2),
+        "ants_per_group": shape[0], "stage_lds": shape[1], "ants_per_wave": shape[2],
+        "roofline": {"bound": "hbm", "kernel": "k_construct_pair" if shape[2] == 2 else "k_construct",
+                     "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": CONSTRUCT_TRAFFIC_BYTES.get(len(seeds)),
@@ -652,7 +655,8 @@ def main():
                    "n": W.C2_N, "K": W.C2_K, "ants": W.C2_ANTS, "cand_k": W.C2_CAND_K,
                    "rho": W.C2_RHO, "seeds": len(seeds),
                    "ants_per_group": head.pop("ants_per_group"),
-                   "stage_lds": head.pop("stage_lds")},
+                   "stage_lds": head.pop("stage_lds"),
+                   "ants_per_wave": head.pop("ants_per_wave")},
         "roofline": head.pop("roofline"),
     }
     out.update(head)

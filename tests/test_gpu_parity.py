@@ -165,12 +165,15 @@ PAIR_CASES = [n for n, (_, kw) in CASES.items() if kw["cand_k"] <= 32]
 
 
 @pytest.mark.parametrize("name", PAIR_CASES)
-@pytest.mark.parametrize("stage,fm,dist", [(1, 1, 0), (0, 1, 0), (0, 0, 0), (0, 1, 1)])
-def test_pair_layout_matches_oracle(gpu, name, stage, fm, dist):
+@pytest.mark.parametrize("stage,fm,dist,ring", [(1, 1, 0, 0), (0, 1, 0, 0), (0, 0, 0, 1),
+                                                (0, 1, 1, 1), (1, 1, 0, 1)])
+def test_pair_layout_matches_oracle(gpu, name, stage, fm, dist, ring, monkeypatch):
     """Two ants per wavefront: four epochs give the oracle's tours, costs,
-    unvisited counts, colony state and pheromone, with
-    rows staged in LDS or read from global memory, with and without the
-    all-pairs fallback weights, in both distance modes."""
+    unvisited counts, colony state and pheromone, with rows staged in LDS or
+    read from global memory, with and without the all-pairs fallback
+    weights, in both distance modes, with the tours built in LDS or in
+    registers (stored out in 128-byte chunks)."""
+    monkeypatch.setenv("ACO_HIP_TOUR_RING", str(ring))
     inst, s, o = make_pair(name, fallback_matrix=fm, dist_mode=dist)
     try:
         s.set_bucket_shape(0, 4, stage, ants_per_wave=2)
@@ -192,10 +195,12 @@ def test_pair_layout_matches_oracle(gpu, name, stage, fm, dist):
     s.close()
 
 
+@pytest.mark.parametrize("ring", [0, 1])
 @pytest.mark.parametrize("ants,apg", [(33, 4), (7, 1), (2, 3), (1, 2)])
-def test_pair_layout_ragged_ant_counts(gpu, ants, apg):
+def test_pair_layout_ragged_ant_counts(gpu, ants, apg, ring, monkeypatch):
     """Odd ant counts leave the last wavefront with one ant; workgroups past
     the colony's ants exit; the tours still equal the one-ant layout's."""
+    monkeypatch.setenv("ACO_HIP_TOUR_RING", str(ring))
     mk, kw = CASES["c1_n100"]
     inst = mk()
     p = Params(**dict(kw, ants=ants))

@@ -503,7 +503,7 @@ __device__ __forceinline__ float pair_scan(float v) {
  * of a wavefront decide in lockstep, one decision each per iteration, and an
  * ant whose tour is complete idles until its partner's is too.  Ant
  * 2(gW + w) + h of the colony is half h of wavefront w of workgroup g. */
-template <bool STAGE, bool STATS, bool XCD, bool FSM>
+template <bool STAGE, bool STATS, bool XCD, bool FSM, bool RING>
 __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per_colony, int W,
                                                         const int *cols) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -577,6 +577,10 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   }
   uint16_t *const s_state0 = reinterpret_cast<uint16_t *>(smem + off) + (size_t)wave * 2 * Sn;
   off += (size_t)W * 2 * Sn * 2;
+  /* without RING, two LDS tour buffers per wave */
+  const size_t tour_elems = round16((size_t)D.stride * 2) / 2;
+  uint16_t *const s_tour0 =
+      RING ? nullptr : reinterpret_cast<uint16_t *>(smem + off) + (size_t)wave * 2 * tour_elems;
   __syncthreads();
   if (STATS && threadIdx.x == 0)
     atomicAdd(P.stats + (size_t)colony * ACO_STATS + 5, clock64() - t_kernel);
@@ -585,18 +589,21 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   if (ant0 >= D.m) return;
   const bool has1 = ant0 + 1 < D.m;
   const int ant_h = ant0 + h;
-  /* this lane's ant's state array; the tours go straight to their output
-   * rows (int32 entries, byte offset hoff + 4 pos of the wave's base), which
-   * keeps a pair's LDS at two state arrays */
+  /* this lane's ant's state array and tour.  Without RING the tour is
+   * built in an LDS buffer and written out at the end.  With RING it goes
+   * to its output row in whole 128-byte chunks (rows are 256-byte aligned):
+   * lane t of a half holds entry 32c + t of the chunk c being filled (entry
+   * 0, the start depot, is 0), and a chunk is stored by its 32 lanes once
+   * complete, so no line is written back half-filled and LDS holds only the
+   * two state arrays. */
   uint16_t *const st_h = s_state0 + (size_t)h * Sn;
-  int32_t *const tour_w = P.tours + D.tour_off + (size_t)ant0 * D.stride;
-  int32_t *const tour_h = tour_w + (size_t)h * D.stride;
-  uint32_t hoff = ((uint32_t)h * (uint32_t)D.stride + 1u) * 4u;
-  asm("" : "+v"(hoff)); /* keeps hoff + 4 pos one shift-add */
+  int32_t *const tour_h = P.tours + D.tour_off + (size_t)ant_h * D.stride;
+  uint16_t *const tr_h = RING ? nullptr : s_tour0 + (size_t)h * tour_elems;
+  int ring = 0;
   const uint16_t *g_dem = P.dem + I.node_off;
   for (int j = t; j < Sn; j += 32)
     st_h[j] = (j == 0 || j >= n1) ? (uint16_t)VISITED : g_dem[j];
-  if (t == 0 && (h == 0 || has1)) tour_h[0] = 0;
+  if (!RING && t == 0) tr_h[0] = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -707,9 +714,11 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     const uint32_t st2 = st_h[j2]; /* before state[move] changes: j2 != move */
     if (__builtin_amdgcn_inverse_ballot_w64(wr_m)) {
       st_h[mvw] = (uint16_t)VISITED;
-      *reinterpret_cast<int32_t *>(reinterpret_cast<char *>(tour_w) +
-                                   (((uint32_t)pos << 2) + hoff)) = (int32_t)mvw;
+      if constexpr (!RING) tr_h[pos + 1] = (uint16_t)mvw;
     }
+    const int wl = (pos + 1) & 31; /* chunk slot of this decision's tour entry */
+    if constexpr (RING)
+      if (__builtin_amdgcn_inverse_ballot_w64(alive_m) && t == wl) ring = (int)mvw;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -721,6 +730,11 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     slack += md;
     pos += __builtin_amdgcn_inverse_ballot_w64(mvm) ? 1 : 0;
     cur = (int)mvw;
+    if constexpr (RING) { /* a move that completes a chunk stores it */
+      const uint64_t fl = mvm & ballot64(wl == 31);
+      if (__builtin_expect(fl != 0ull, 0))
+        if (__builtin_amdgcn_inverse_ballot_w64(fl)) tour_h[(pos & ~31) + t] = ring;
+    }
     /* aco_cuda_kernels.cu:492: the route goes on while customers remain,
      * load < Q and no roulette closed it */
     const uint64_t cm = alive_m & (~mvm | ballot64(pos == lim) | ballot64(room == 0));
@@ -729,11 +743,24 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
        * the depot; with no customer left the remaining routes are empty */
       if (__builtin_amdgcn_inverse_ballot_w64(cm)) {
         const int zeros = pos == lim ? K - veh : 1;
-        for (int z = t; z < zeros; z += 32) tour_h[pos + 1 + z] = 0;
+        if constexpr (RING) {
+          /* entries pos+1 .. pos+zeros are 0; every chunk they complete is stored */
+          const int p0 = pos + 1, p1 = pos + zeros;
+          for (int c0 = p0 & ~31;; c0 += 32) {
+            const int e = c0 + t;
+            if (e >= p0 && e <= p1) ring = 0;
+            if (c0 + 31 > p1) break;
+            tour_h[e] = ring;
+          }
+        } else {
+          for (int z = t; z < zeros; z += 32) tr_h[pos + 1 + z] = 0;
+        }
         pos += zeros;
         veh += zeros;
         lim += zeros;
         if (veh >= K) {
+          if constexpr (RING) /* the last, partial chunk */
+            if (t <= (pos & 31)) tour_h[(pos & ~31) + t] = ring;
           dec = s;
           if (STATS) t_done = clock64();
         } else {
@@ -756,17 +783,29 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     }
   }
   const int rem_cnt = lim - pos;
-  /* the tour stores are complete and visible to the wave's loads (the rows
-   * were never read before, so no stale L1 line holds them) */
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  /* sum both tours' edge costs (int64 cost units) */
   const bool mine = h == 0 || has1;
   long long cost = 0;
-  if (mine) {
-    for (int p = t + 1; p <= pos; p += 32)
-      cost += edge_units(P, c_xy[tour_h[p - 1]], c_xy[tour_h[p]]);
+  if constexpr (RING) {
+    /* the tour stores are complete and visible to the wave's loads (the rows
+     * were never read before, so no stale L1 line holds them) */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    /* sum both tours' edge costs (int64 cost units) */
+    if (mine)
+      for (int p = t + 1; p <= pos; p += 32)
+        cost += edge_units(P, c_xy[tour_h[p - 1]], c_xy[tour_h[p]]);
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    /* write both tours out and sum their edge costs (int64 cost units) */
+    if (mine)
+      for (int p = t; p <= pos; p += 32) {
+        const int v = tr_h[p];
+        tour_h[p] = v;
+        if (p >= 1) cost += edge_units(P, c_xy[tr_h[p - 1]], c_xy[v]);
+      }
   }
   for (int o = 16; o > 0; o >>= 1) cost += __shfl_xor(cost, o, 64);
   if (t == 0 && mine) {

@@ -17,9 +17,13 @@ ACO_CONSTRUCT_VARIANTS(3)
 ACO_CONSTRUCT_VARIANTS(4)
 #undef ACO_CONSTRUCT_VARIANTS
 
-template __global__ void k_construct_pair<true, false, false, false>(DevCtx, int, int, const int *);
-template __global__ void k_construct_pair<false, false, false, false>(DevCtx, int, int, const int *);
-template __global__ void k_construct_pair<true, true, false, false>(DevCtx, int, int, const int *);
-template __global__ void k_construct_pair<false, true, false, false>(DevCtx, int, int, const int *);
-template __global__ void k_construct_pair<true, false, true, false>(DevCtx, int, int, const int *);
-template __global__ void k_construct_pair<false, false, true, false>(DevCtx, int, int, const int *);
+#define ACO_PAIR_VARIANTS(RG)                                                              \
+  template __global__ void k_construct_pair<true, false, false, false, RG>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct_pair<false, false, false, false, RG>(DevCtx, int, int, const int *); \
+  template __global__ void k_construct_pair<true, true, false, false, RG>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct_pair<false, true, false, false, RG>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct_pair<true, false, true, false, RG>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct_pair<false, false, true, false, RG>(DevCtx, int, int, const int *);
+ACO_PAIR_VARIANTS(false)
+ACO_PAIR_VARIANTS(true)
+#undef ACO_PAIR_VARIANTS

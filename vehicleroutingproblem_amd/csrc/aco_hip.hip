@@ -97,12 +97,13 @@ static int fs_stride_of(int n1) { return (n1 + 15) & ~15; }
  * entries), depot scores and coordinates; the computed-fallback tables
  * (taupow, and eta unless eta_len == 0) when the weight matrix is off; per
  * wave a state array of S u16, a tour buffer of `stride` u16 and a 128-byte
- * store sink, or with `pair` two state arrays (the pair writes its tours to
- * global memory). */
+ * store sink, or with `pair` two state arrays and (unless `ring`: the pair
+ * then stores its tours in 128-byte chunks from registers) two tour buffers. */
 static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on, int stride,
-                      int pair) {
+                      int pair, int ring = 0) {
   const size_t state = (size_t)fs_stride_of(n1) * 2;
-  size_t b = (size_t)W * (pair ? 2 * state : state + r16((size_t)stride * 2) + 128);
+  const size_t tour = r16((size_t)stride * 2);
+  size_t b = (size_t)W * (pair ? 2 * state + (ring ? 0 : 2 * tour) : state + tour + 128);
   if (!fs_on) b += 256 * 4 + r16((size_t)eta_len * 4);
   if (stage) {
     const size_t ent = (size_t)n1 * k + 64;
@@ -244,12 +245,30 @@ static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_
   int W = std::max(1, std::min(8, std::min(W_req, (c->prm.ants + apw - 1) / apw)));
   int stage = want_stage == 0 ? 0 : 1;
   int eta_in_lds = 1;
+  int ring = 0;
   size_t need = 0;
+  /* pairs keep their tours in LDS when the workgroups a CU holds at once
+   * (the bucket's share of the device, at most 32 wavefronts) all fit with
+   * them; otherwise in registers, stored out in 128-byte chunks */
   auto recompute = [&]() {
-    need = 0;
-    for (int i : lb.insts)
-      need = std::max(need, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0, W,
-                                    stage, c->fs_on, c->max_stride, pair));
+    auto need_with = [&](int rg) {
+      size_t b = 0;
+      for (int i : lb.insts)
+        b = std::max(b, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0, W,
+                                stage, c->fs_on, c->max_stride, pair, rg));
+      return b;
+    };
+    ring = 0;
+    need = need_with(0);
+    if (pair) {
+      const long long groups = (long long)lb.cols.size() * ((c->prm.ants + 2 * W - 1) / (2 * W));
+      const long long per_cu = std::max(1LL, std::min((groups + c->num_cu - 1) / c->num_cu,
+                                                      (long long)(32 / W)));
+      if (c->ring_mode > 0 || (c->ring_mode < 0 && (long long)need * per_cu > (long long)LDS_LIMIT)) {
+        ring = 1;
+        need = need_with(1);
+      }
+    }
   };
   recompute();
   if (need > LDS_LIMIT) {
@@ -289,6 +308,7 @@ static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_
   lb.eta_lds = eta_in_lds;
   lb.rows = (kmax + 15) / 16;
   lb.pair = pair;
+  lb.ring = ring;
   lb.groups = (c->prm.ants + W * apw - 1) / (W * apw);
   lb.lds = need;
   return ACO_HIP_OK;
@@ -487,6 +507,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     }
   }
   c->pair_mode = env_int("ACO_HIP_PAIR", -1);
+  c->ring_mode = env_int("ACO_HIP_TOUR_RING", -1);
   const int W0 = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
   int want_stage = p->stage_lds;
   if (want_stage < 0) want_stage = env_int("ACO_HIP_STAGE_LDS", -1);
@@ -673,17 +694,18 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       (const void *)k_construct<false, true, R, false, F>,                                     \
       (const void *)k_construct<true, false, R, true, F>,                                      \
       (const void *)k_construct<false, false, R, true, F>
-#define ACO_PAIR_VARIANTS(F)                                                                    \
-  (const void *)k_construct_pair<true, false, false, F>,                                       \
-      (const void *)k_construct_pair<false, false, false, F>,                                  \
-      (const void *)k_construct_pair<true, true, false, F>,                                    \
-      (const void *)k_construct_pair<false, true, false, F>,                                   \
-      (const void *)k_construct_pair<true, false, true, F>,                                    \
-      (const void *)k_construct_pair<false, false, true, F>
+#define ACO_PAIR_VARIANTS(F, RG)                                                                \
+  (const void *)k_construct_pair<true, false, false, F, RG>,                                   \
+      (const void *)k_construct_pair<false, false, false, F, RG>,                              \
+      (const void *)k_construct_pair<true, true, false, F, RG>,                                \
+      (const void *)k_construct_pair<false, true, false, F, RG>,                               \
+      (const void *)k_construct_pair<true, false, true, F, RG>,                                \
+      (const void *)k_construct_pair<false, false, true, F, RG>
   const void *variants[] = {ACO_VARIANTS(1, true),  ACO_VARIANTS(2, true),  ACO_VARIANTS(3, true),
                             ACO_VARIANTS(4, true),  ACO_VARIANTS(1, false), ACO_VARIANTS(2, false),
-                            ACO_VARIANTS(3, false), ACO_VARIANTS(4, false), ACO_PAIR_VARIANTS(true),
-                            ACO_PAIR_VARIANTS(false)};
+                            ACO_VARIANTS(3, false), ACO_VARIANTS(4, false),
+                            ACO_PAIR_VARIANTS(true, false), ACO_PAIR_VARIANTS(false, false),
+                            ACO_PAIR_VARIANTS(true, true), ACO_PAIR_VARIANTS(false, true)};
 #undef ACO_VARIANTS
 #undef ACO_PAIR_VARIANTS
   for (const void *f : variants)
@@ -810,11 +832,11 @@ extern "C" int aco_hip_init_tau(AcoHipContext *c) {
 }
 
 /* One launch of the construction variant that matches the bucket's shape:
- * ANTS = 1 (k_construct, ROWS = R) or 2 (k_construct_pair), FSM = fallback
- * weights from the all-pairs weight matrix.  Launches of several colonies
- * keep each colony's workgroups on one XCD (the counter variants keep the
- * plain order). */
-template <int ANTS, int R, bool FSM>
+ * ANTS = 1 (k_construct, ROWS = R) or 2 (k_construct_pair, tours in
+ * registers with RING), FSM = fallback weights from the all-pairs weight
+ * matrix.  Launches of several colonies keep each colony's workgroups on one
+ * XCD (the counter variants keep the plain order). */
+template <int ANTS, int R, bool FSM, bool RING = false>
 static void launch_variant(const AcoHipContext *c, const LaunchBucket &lb, hipStream_t st,
                            bool stats) {
   DevCtx P = c->P;
@@ -825,7 +847,7 @@ static void launch_variant(const AcoHipContext *c, const LaunchBucket &lb, hipSt
 #define ACO_GO(ST, SS, X)                                                                       \
   do {                                                                                          \
     if constexpr (ANTS == 2)                                                                    \
-      hipLaunchKernelGGL((k_construct_pair<ST, SS, X, FSM>), grid, block, lb.lds, st, P,        \
+      hipLaunchKernelGGL((k_construct_pair<ST, SS, X, FSM, RING>), grid, block, lb.lds, st, P,  \
                          lb.groups, lb.W, cols);                                                \
     else                                                                                        \
       hipLaunchKernelGGL((k_construct<ST, SS, R, X, FSM>), grid, block, lb.lds, st, P,          \
@@ -850,7 +872,10 @@ template <bool FSM>
 static void launch_bucket_f(const AcoHipContext *c, const LaunchBucket &lb, hipStream_t st,
                             bool stats) {
   if (lb.pair) {
-    launch_variant<2, 1, FSM>(c, lb, st, stats);
+    if (lb.ring)
+      launch_variant<2, 1, FSM, true>(c, lb, st, stats);
+    else
+      launch_variant<2, 1, FSM, false>(c, lb, st, stats);
     return;
   }
   switch (lb.rows) {

@@ -180,7 +180,7 @@ __global__ void k_knn(DevCtx P, int total_rows, int knn_key, float depot_w);
 __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
 template <bool STAGE, bool STATS, int ROWS, bool XCD, bool FSM>
 __global__ void k_construct(DevCtx P, int groups_per_colony, int W, const int *cols);
-template <bool STAGE, bool STATS, bool XCD, bool FSM>
+template <bool STAGE, bool STATS, bool XCD, bool FSM, bool RING>
 __global__ void k_construct_pair(DevCtx P, int groups_per_colony, int W, const int *cols);
 __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
 __global__ void k_evaporate_flat(DevCtx P, const long long *item_off, const long long *slot_off,
