@@ -145,34 +145,38 @@ def test_applied_prior_matches_oracle_solver(gpu):
 
 
 def test_learned_prior_gains_over_plain_aco(gpu):
-    """What the trained network gains: elite tours of 32 seeded training
-    instances (n in [200, 2000]) train the prior; on two n=2000 test
-    instances the learned heuristic reaches a lower best cost than plain ACO
-    within the same wall-clock budget (the prior's feature pass and
-    inference are inside the learned run's budget)."""
+    """What the trained network gains, at equal wall-clock.  Training: the
+    kNN-20 candidate edges of 512 seeded instances (n in [200, 2000], half
+    uniform, half clustered), labelled by their elite tours (plain ACO, 64
+    ants, 100 epochs), 600 full-batch Adam steps.  Test: eight n=2000 K=16
+    instances (256 ants, k=16; even index uniform, odd clustered sigma 30) in
+    one batched context for 2 s, once plain and once with the prior applied
+    to every instance (its feature pass and inference inside the budget).
+    The mean best-cost gain is at least 2%."""
     import time
+    from vehicleroutingproblem_amd import workloads as W
     from vehicleroutingproblem_amd.prior import train_prior, training_instances
-    pr = train_prior(training_instances(32, seed=3), epochs=100, steps=400)
-    gains = []
-    for t in range(2):
-        inst = (Instance.uniform(2000, 16, side=1000, dmin=1, dmax=10, slack_pct=110, seed=900 + t)
-                if t == 0 else
-                Instance.clustered(2000, 16, side=1000, clusters=25, sigma=30, dmin=1, dmax=10,
-                                   slack_pct=110, seed=900 + t))
-        costs = {}
-        for learned in (False, True):
-            with HipSolver(inst, Params(cand_k=16, ants=256, rho=0.1), seeds=[41 + t]) as s:
-                s.prepare()
-                t0 = time.perf_counter()
-                if learned:
-                    pr.apply(s)
-                s.run(timeout_seconds=1.5 - (time.perf_counter() - t0), max_epochs=100000,
-                      chunk_epochs=16)
-                costs[learned] = s.best()[1]
-        gains.append((costs[False] - costs[True]) / costs[False])
+    pr = train_prior(training_instances(512, seed=7), epochs=100, steps=0, batch=512,
+                     cand_k=20)
+    pr.train(600, lr=2e-3)
+    test = [W.c5_instance(t) for t in range(8)]
+    seeds = [77 + t for t in range(len(test))]
+    best = {}
+    for learned in (False, True):
+        with HipSolver(test, Params(cand_k=16, ants=256, rho=0.1), seeds=seeds) as s:
+            s.prepare()
+            s.sync()
+            t0 = time.perf_counter()
+            if learned:
+                for i in range(len(test)):
+                    pr.apply(s, instance=i)
+            s.run(timeout_seconds=2.0 - (time.perf_counter() - t0), max_epochs=10 ** 6,
+                  chunk_epochs=16)
+            best[learned] = [s.best(i)[1] for i in range(len(test))]
     pr.close()
+    gains = [(a - b) / a for a, b in zip(best[False], best[True])]
     print("learned-prior gain at equal wall-clock:", [round(100 * g, 2) for g in gains], "%")
-    assert np.mean(gains) > 0.0, gains
+    assert np.mean(gains) >= 0.02, gains
 
 
 def test_prior_in_a_mixed_batch_matches_single_runs(gpu):

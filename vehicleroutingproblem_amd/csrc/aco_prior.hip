@@ -5,8 +5,11 @@
  * (aco_cuda_kernels.cu:295).  The network has no reference counterpart.
  *
  * Network: 8 edge features -> 64 -> 64 -> 1 logit, ReLU, BF16 weights and
- * activations into v_mfma_f32_16x16x16bf16_1k, FP32 accumulation, FP32
- * master weights with Adam.  Trained on the GPU by the kernels below with a
+ * activations into the matrix cores, FP32 accumulation, FP32 master weights
+ * with Adam.  The products over the 64 hidden units (layer 2 and dH1 =
+ * dZ2 W2) use v_mfma_f32_16x16x32_bf16, two K=16 slices per instruction;
+ * layer 1 (K = 8 features, padded to 16) and the products over a tile's 16
+ * edges (dW2, dW1) use v_mfma_f32_16x16x16bf16_1k.  Trained on the GPU by the kernels below with a
  * mean binary cross-entropy against "the edge joins consecutive nodes of an
  * elite tour".  Installed as eta = (1/d)^beta * gain * sigmoid(logit), so the
  * candidate, depot-close and fallback weights stay on one scale.
@@ -88,6 +91,20 @@ __device__ __forceinline__ bf4 pack4(float a, float b, float c, float d) {
 
 __device__ __forceinline__ fl4 mfma16(bf4 a, bf4 b, fl4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef short sh8 __attribute__((ext_vector_type(8)));
+
+/* v_mfma_f32_16x16x32_bf16 (gfx950, twice the flops per cycle of the
+ * 16x16x16 form) on two K=16 slices at once: lane l's A values are
+ * (a0, a1) and its B values (b0, b1), so k-slot i of a0 meets k-slot i of
+ * b0 and so on; any such pairing of two 16x16x16 products gives their sum. */
+__device__ __forceinline__ fl4 mfma32(bf4 a0, bf4 a1, bf4 b0, bf4 b1, fl4 c) {
+  const sh8 a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+  const sh8 b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                 __builtin_bit_cast(bf8, b), c, 0, 0, 0);
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -277,9 +294,11 @@ __device__ __forceinline__ void tile_forward(const LaneW &W, bf4 xa, const uint1
     const float b = W.b2[nt];
     fl4 acc = {b, b, b, b};
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-      acc = mfma16(ha[ks], *reinterpret_cast<const bf4 *>(&w2[(16 * nt + c) * MLP_HS + 16 * ks + 4 * g]),
-                   acc);
+    for (int ks = 0; ks < 4; ks += 2) {
+      const uint16_t *wr = &w2[(16 * nt + c) * MLP_HS + 16 * ks + 4 * g];
+      acc = mfma32(ha[ks], ha[ks + 1], *reinterpret_cast<const bf4 *>(wr),
+                   *reinterpret_cast<const bf4 *>(wr + 16), acc);
+    }
     z2[nt] = acc;
   }
   fl4 s = {0, 0, 0, 0};
@@ -427,10 +446,11 @@ __global__ void __launch_bounds__(64 * GRAD_WAVES, 2)
     for (int nt = 0; nt < 4; ++nt) {
       fl4 dh = {0, 0, 0, 0};
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks)
-        dh = mfma16(da[ks],
-                    *reinterpret_cast<const bf4 *>(&L.m.w2t[(16 * nt + c) * MLP_HS + 16 * ks + 4 * g]),
-                    dh);
+      for (int ks = 0; ks < 4; ks += 2) {
+        const uint16_t *wr = &L.m.w2t[(16 * nt + c) * MLP_HS + 16 * ks + 4 * g];
+        dh = mfma32(da[ks], da[ks + 1], *reinterpret_cast<const bf4 *>(wr),
+                    *reinterpret_cast<const bf4 *>(wr + 16), dh);
+      }
       fl4 dz1;
       float sb = 0.0f;
 #pragma unroll
