@@ -459,8 +459,9 @@ def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int =
         r_ = HipSolver(insts, p, seeds=seeds)
         r_.prepare()
         if warmup > 0:
-            r_.run(stagnation_epochs=80, max_epochs=warmup,
-                   chunk_epochs=div_chunk(warmup, chunk_cap), use_graph=True)
+            # the timed run's chunk length, so its epoch graph is captured here: the
+            # epochs of a chunk past max_epochs find their colonies halted and return
+            r_.run(stagnation_epochs=80, max_epochs=warmup, chunk_epochs=chunk, use_graph=True)
         r_.sync()
         return r_
 

@@ -381,11 +381,17 @@ void oracle_set_tau(OracleColony *o, const uint8_t *tau) {
 
 void oracle_set_epoch(OracleColony *o, int epoch) { o->st.epoch = epoch; }
 
+/* New stop rules start a new run of the colony: its halted flag is
+ * re-evaluated under them, as the product's run loop does when a colony is
+ * continued by a later run (the reference's solve is one-shot). */
 void oracle_set_stop_rules(OracleColony *o, int stagnation_epochs, double min_rel,
                            int fixed_epochs) {
   o->stagnation = stagnation_epochs;
   o->min_rel = min_rel;
   o->fixed_epochs = fixed_epochs;
+  OracleState *st = &o->st;
+  if (fixed_epochs > 0) st->halted = st->epoch >= fixed_epochs;
+  else st->halted = stagnation_epochs > 0 && st->iter_since_best >= stagnation_epochs;
 }
 
 /* ---- tour construction (aco_cuda_kernels.cu:447-562) --------------------- */

@@ -384,6 +384,34 @@ def test_consecutive_runs_continue(gpu):
     s.close()
 
 
+def test_graph_survives_a_change_of_stop_rules(gpu):
+    """Runs with one chunk length replay one captured epoch graph while their
+    stop rules change (the rules are read from device memory at each epoch's
+    end): fixed epochs 4, then an epoch cap of 12, then a stagnation stop of 3
+    epochs under a cap of 60.  Epoch counts, halting and the colony equal the
+    oracle under the same sequence of rules."""
+    inst, s, o = make_pair("c1_n100")
+    s.prepare()
+    assert s.run(fixed_epochs=4, chunk_epochs=4)["epochs"] == 4
+    assert s.run(max_epochs=12, chunk_epochs=4)["epochs"] == 8
+    o.set_stop_rules(fixed=12)
+    o.epochs(12)
+    assert s.colony_state()["epoch"] == 12
+    assert s.colony_state()["global_best_cost"] == o.state()["global_best_cost"]
+    got = s.run(stagnation_epochs=3, max_epochs=60, chunk_epochs=4)["epochs"]
+    o.set_stop_rules(stagnation=3)
+    want = 0
+    while not o.state()["halted"] and want < 48:
+        o.epochs(1)
+        want += 1
+    gs, os_ = s.colony_state(), o.state()
+    assert got == want and gs["epoch"] == os_["epoch"], (got, want)
+    for key in ("iter_since_best", "global_best_cost", "best_ant", "halted"):
+        assert gs[key] == os_[key], key
+    assert np.array_equal(s.tau()[0], o.tau()[0])
+    s.close()
+
+
 def test_batch_matches_single(gpu):
     """Instances batched in one context with two colonies each give the
     tours of each instance run alone (the Philox stream depends only on the

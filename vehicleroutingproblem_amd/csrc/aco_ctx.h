@@ -84,6 +84,8 @@ struct AcoHipContext {
   HistEntry *d_hist = nullptr;
   unsigned long long *d_stats = nullptr;
   int *d_done = nullptr;
+  RunRules *d_rules = nullptr; /* the current run's stop rules (read by the select kernels) */
+  RunRules rules{};            /* host copy of *d_rules */
   DevCtx P{};
   /* timing */
   bool timing = false;

@@ -58,7 +58,7 @@ static void free_pools(AcoHipContext *c) {
                   c->d_dscore, c->d_rec, c->d_pher,   c->d_tours,    c->d_len,  c->d_unvis,
                   c->d_best, c->d_cost,  c->d_deq,    c->d_thr,      c->d_taupow, c->d_eta,
                   c->d_scratch_tour, c->d_hist, c->d_stats, c->d_cidx, c->d_score, c->d_done, c->d_slot,
-                  c->d_fscore, c->d_ceta};
+                  c->d_fscore, c->d_ceta, c->d_rules};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->d_inst = nullptr; c->d_col = nullptr; c->d_state = nullptr; c->d_row_inst = nullptr;
@@ -67,7 +67,7 @@ static void free_pools(AcoHipContext *c) {
   c->d_best = nullptr; c->d_cost = nullptr; c->d_deq = nullptr; c->d_thr = nullptr;
   c->d_taupow = nullptr; c->d_eta = nullptr; c->d_scratch_tour = nullptr; c->d_hist = nullptr;
   c->d_stats = nullptr; c->d_cidx = nullptr; c->d_score = nullptr; c->d_done = nullptr;
-  c->d_slot = nullptr; c->d_fscore = nullptr; c->d_ceta = nullptr;
+  c->d_slot = nullptr; c->d_fscore = nullptr; c->d_ceta = nullptr; c->d_rules = nullptr;
   if (c->d_colmap) (void)hipFree(c->d_colmap);
   c->d_colmap = nullptr;
   if (c->d_feas) (void)hipFree(c->d_feas);
@@ -546,6 +546,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   if ((rc = dalloc(c, &c->d_stats, sizeof(unsigned long long) * ACO_STATS * c->num_col)))
     return rc;
   if ((rc = dalloc(c, &c->d_done, sizeof(int) * c->num_col))) return rc;
+  if ((rc = dalloc(c, &c->d_rules, sizeof(RunRules)))) return rc;
   if ((rc = dalloc(c, &c->d_colmap, sizeof(int) * c->num_col))) return rc;
   if ((rc = dalloc(c, &c->d_feas, sizeof(int) * c->num_col))) return rc;
   if ((rc = dalloc(c, &c->d_item_off, sizeof(long long) * (c->num_col + 1)))) return rc;
@@ -679,10 +680,9 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   P.dist_real = p->dist_mode == ACO_HIP_DIST_REAL ? 1 : 0;
   P.q_evap = c->tables.q_evap;
   P.qdep = p->Q;
-  P.fixed_epochs = 0;
-  P.stagnation = 0;
-  P.max_epochs = 0;
-  P.min_rel = 1e-3;
+  c->rules = RunRules{0, 0, 0, 0, 1e-3}; /* no stop rule outside a run */
+  HIPC(c, hipMemcpy(c->d_rules, &c->rules, sizeof(RunRules), hipMemcpyHostToDevice));
+  P.rules = c->d_rules;
   P.hist = (void *)c->d_hist;
   P.stats = nullptr;
   P.done = c->d_done;
@@ -1201,17 +1201,19 @@ extern "C" int aco_hip_run(AcoHipContext *c, const AcoHipRunConfig *cfg, AcoHipR
   NEED_LOADED(c);
   if (!cfg) return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "null run config");
   HIPC(c, hipSetDevice(c->device));
-  c->P.fixed_epochs = cfg->fixed_epochs > 0 ? cfg->fixed_epochs : 0;
-  c->P.stagnation = cfg->stagnation_epochs > 0 ? cfg->stagnation_epochs : 0;
-  c->P.max_epochs = cfg->max_epochs > 0 ? cfg->max_epochs : 0;
-  c->P.min_rel = cfg->min_rel_improvement > 0.0 ? cfg->min_rel_improvement : 1e-3;
-  if (!c->P.fixed_epochs && !c->P.stagnation && !c->P.max_epochs && !(cfg->timeout_seconds > 0))
+  RunRules R{};
+  R.fixed_epochs = cfg->fixed_epochs > 0 ? cfg->fixed_epochs : 0;
+  R.stagnation = cfg->stagnation_epochs > 0 ? cfg->stagnation_epochs : 0;
+  R.max_epochs = cfg->max_epochs > 0 ? cfg->max_epochs : 0;
+  R.min_rel = cfg->min_rel_improvement > 0.0 ? cfg->min_rel_improvement : 1e-3;
+  if (!R.fixed_epochs && !R.stagnation && !R.max_epochs && !(cfg->timeout_seconds > 0))
     return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
                    "no stop rule: set a timeout, stagnation, fixed or max epochs");
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
+  /* the rules live in device memory, so an epoch graph captured by an earlier run with
+   * the same chunk length stays valid */
+  c->rules = R;
+  HIPC(c, hipMemcpyAsync(c->d_rules, &c->rules, sizeof(RunRules), hipMemcpyHostToDevice,
+                         c->stream));
   int chunk = cfg->chunk_epochs > 0 ? std::min(cfg->chunk_epochs, HIST) : 8;
   const bool use_graph = cfg->use_graph && !c->timing;
   std::vector<AcoHipColonyState> st(c->num_col);
@@ -1221,9 +1223,9 @@ extern "C" int aco_hip_run(AcoHipContext *c, const AcoHipRunConfig *cfg, AcoHipR
                     hipMemcpyDeviceToHost));
   /* a new run re-evaluates the halted flag under its own stop rules */
   for (AcoHipColonyState &s : st) {
-    if (c->P.fixed_epochs > 0) s.halted = s.epoch >= c->P.fixed_epochs;
-    else s.halted = c->P.stagnation > 0 && s.iter_since_best >= c->P.stagnation;
-    if (c->P.max_epochs > 0 && s.epoch >= c->P.max_epochs) s.halted = 1;
+    if (R.fixed_epochs > 0) s.halted = s.epoch >= R.fixed_epochs;
+    else s.halted = R.stagnation > 0 && s.iter_since_best >= R.stagnation;
+    if (R.max_epochs > 0 && s.epoch >= R.max_epochs) s.halted = 1;
   }
   HIPC(c, hipMemcpy(c->d_state, st.data(), sizeof(AcoHipColonyState) * c->num_col,
                     hipMemcpyHostToDevice));

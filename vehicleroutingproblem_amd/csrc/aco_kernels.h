@@ -69,6 +69,16 @@ struct ColonyDesc {
   long long fs_off;    /* float elements into fscore (multiple of 16) */
 };
 
+/* Stop rules of a run (aco_cuda.cu:322-399): fixed epochs, stagnation count,
+ * an epoch cap and the significant-improvement threshold. */
+struct RunRules {
+  int fixed_epochs;
+  int stagnation;
+  int max_epochs;
+  int pad;
+  double min_rel;
+};
+
 struct DevCtx {
   const InstDesc *inst;
   const ColonyDesc *col;
@@ -98,10 +108,8 @@ struct DevCtx {
   int dist_real;       /* ACO_HIP_DIST_REAL: float distances, costs in 2^-23 units */
   uint32_t q_evap;
   float qdep;
-  int fixed_epochs;
-  int stagnation;
-  int max_epochs;
-  double min_rel;
+  const RunRules *rules; /* stop rules of the current run, read at each epoch's end (device
+                          * memory, so a captured epoch graph outlives a change of rules) */
   void *hist;          /* HistEntry[colonies][ACO_HIST] per-epoch progress ring */
   unsigned long long *stats; /* [colonies][ACO_STATS] counters, NULL = off */
   int *done;           /* per-colony block counter of k_evap_update (kept at 0) */

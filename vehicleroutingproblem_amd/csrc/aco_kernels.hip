@@ -520,7 +520,7 @@ __device__ bool select_best_block(const DevCtx &P, int colony) {
         const double prev =
             S->global_best_cost == INT64_MAX ? 1.7976931348623157e308 : (double)S->global_best_cost;
         S->improved = 1;
-        if (aco_significant(prev, (double)c, P.min_rel)) S->iter_since_best = 0;
+        if (aco_significant(prev, (double)c, P.rules->min_rel)) S->iter_since_best = 0;
         else S->iter_since_best++;
         S->global_best_cost = c;
         S->best_len = P.len[D.ant_off + ant];
@@ -538,9 +538,10 @@ __device__ bool select_best_block(const DevCtx &P, int colony) {
       S->iter_since_best++;
     }
     S->epoch++;
-    if (P.fixed_epochs > 0) S->halted = S->epoch >= P.fixed_epochs;
-    else if (P.stagnation > 0 && S->iter_since_best >= P.stagnation) S->halted = 1;
-    if (P.max_epochs > 0 && S->epoch >= P.max_epochs) S->halted = 1;
+    const RunRules R = *P.rules;
+    if (R.fixed_epochs > 0) S->halted = S->epoch >= R.fixed_epochs;
+    else if (R.stagnation > 0 && S->iter_since_best >= R.stagnation) S->halted = 1;
+    if (R.max_epochs > 0 && S->epoch >= R.max_epochs) S->halted = 1;
     HistEntry *h = reinterpret_cast<HistEntry *>(P.hist) + (size_t)colony * ACO_HIST +
                    (S->epoch - 1) % ACO_HIST;
     h->epoch = S->epoch;
@@ -881,7 +882,7 @@ __global__ void __launch_bounds__(256) k_select_deposit(DevCtx P, int stage_eta,
       if (improved) {
         const double prev =
             gbest == INT64_MAX ? 1.7976931348623157e308 : (double)gbest;
-        if (aco_significant(prev, (double)c, P.min_rel)) S->iter_since_best = 0;
+        if (aco_significant(prev, (double)c, P.rules->min_rel)) S->iter_since_best = 0;
         else S->iter_since_best++;
         S->global_best_cost = c;
         S->best_len = len1;
@@ -897,9 +898,10 @@ __global__ void __launch_bounds__(256) k_select_deposit(DevCtx P, int stage_eta,
       S->iter_since_best++;
     }
     S->epoch++;
-    if (P.fixed_epochs > 0) S->halted = S->epoch >= P.fixed_epochs;
-    else if (P.stagnation > 0 && S->iter_since_best >= P.stagnation) S->halted = 1;
-    if (P.max_epochs > 0 && S->epoch >= P.max_epochs) S->halted = 1;
+    const RunRules R = *P.rules;
+    if (R.fixed_epochs > 0) S->halted = S->epoch >= R.fixed_epochs;
+    else if (R.stagnation > 0 && S->iter_since_best >= R.stagnation) S->halted = 1;
+    if (R.max_epochs > 0 && S->epoch >= R.max_epochs) S->halted = 1;
     HistEntry *h = reinterpret_cast<HistEntry *>(P.hist) + (size_t)colony * ACO_HIST +
                    (S->epoch - 1) % ACO_HIST;
     h->epoch = S->epoch;
