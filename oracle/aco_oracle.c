@@ -684,12 +684,59 @@ void oracle_deposit_tour(OracleColony *o, const int32_t *tour, int len, float de
   const size_t W = (size_t)o->n + 1;
   for (int p = 0; p + 1 < len; ++p) {
     int u = tour[p], v = tour[p + 1];
+    if ((unsigned)u > (unsigned)o->n || (unsigned)v > (unsigned)o->n) continue;
     uint8_t a = o->tau[(size_t)u * W + v];
     uint8_t nq = oracle_quantize(o, o->deq[a] + delta);
     if (o->tau[(size_t)u * W + v] < nq) o->tau[(size_t)u * W + v] = nq;
     if (o->tau[(size_t)v * W + u] < nq) o->tau[(size_t)v * W + u] = nq;
     patch_cand(o, u, v);
     patch_cand(o, v, u);
+  }
+}
+
+/* The reference's per-route deposits, each route in turn and its edges in
+ * order (the K threads of kernel_deposit_solution / kernel_deposit_flat_solution,
+ * aco_cuda_kernels.cu:592-668, one after the other).  step_major: node pos
+ * of route v is routes[(off_v + pos) * m + ant], off_v = sum over u < v of
+ * (lens[ant*K + u] - 1) (:603-612); else routes[v * (n+1) + pos] with lens[v]
+ * (:650-656).  Every route starts from the depot (prev = 0).  A position
+ * outside the buffer (steps rows, or K rows of n+1) or a node outside 0..n
+ * reads as "no node": edges touching it deposit nothing. */
+void oracle_deposit_routes(OracleColony *o, const int32_t *routes, const int32_t *lens, int K,
+                           int m, long long steps, int ant, float delta, int step_major) {
+  const size_t W = (size_t)o->n + 1;
+  const long long limit = step_major ? steps : (long long)K * (long long)W;
+  long long off = 0;
+  for (int v = 0; v < K; ++v) {
+    const int len = step_major ? lens[(size_t)ant * K + v] : lens[v];
+    const long long base = step_major ? off : (long long)v * (long long)W;
+    int prev = 0;
+    for (int pos = 1; pos < len; ++pos) {
+      const long long e = base + pos;
+      int cur = -1;
+      if (e >= 0 && e < limit) cur = step_major ? routes[e * m + ant] : routes[e];
+      if ((unsigned)cur > (unsigned)o->n) cur = -1;
+      if (prev >= 0 && cur >= 0) {
+        const uint8_t a = o->tau[(size_t)prev * W + cur];
+        const uint8_t nq = oracle_quantize(o, o->deq[a] + delta);
+        if (o->tau[(size_t)prev * W + cur] < nq) o->tau[(size_t)prev * W + cur] = nq;
+        if (o->tau[(size_t)cur * W + prev] < nq) o->tau[(size_t)cur * W + prev] = nq;
+        patch_cand(o, prev, cur);
+        patch_cand(o, cur, prev);
+      }
+      prev = cur;
+    }
+    off += (long long)len - 1;
+  }
+}
+
+/* kernel_evaporate_tau (aco_cuda_kernels.cu:145-155) with any delta and
+ * q_min: every byte of the matrix, q > q_min + delta ? q - delta : q_min. */
+void oracle_evaporate_levels(OracleColony *o, uint8_t delta, uint8_t q_min) {
+  const size_t nn = (size_t)(o->n + 1) * (o->n + 1);
+  for (size_t e = 0; e < nn; ++e) {
+    const uint8_t q = o->tau[e];
+    o->tau[e] = (q > q_min + delta) ? (uint8_t)(q - delta) : q_min;
   }
 }
 

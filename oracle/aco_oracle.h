@@ -55,6 +55,13 @@ void oracle_set_ant_results(OracleColony *o, const int64_t *cost, const int32_t 
                             const int32_t *len);
 void oracle_evaporate(OracleColony *o);
 void oracle_deposit_tour(OracleColony *o, const int32_t *tour, int len, float delta);
+/* Per-route deposits of the reference's route layouts on the colony's
+ * matrix (step_major 1: kernel_deposit_solution's routes[steps][m] and
+ * lens[m][K]; 0: kernel_deposit_flat_solution's routes[K][n+1] and lens[K]). */
+void oracle_deposit_routes(OracleColony *o, const int32_t *routes, const int32_t *lens, int K,
+                           int m, long long steps, int ant, float delta, int step_major);
+/* kernel_evaporate_tau with explicit delta and q_min (whole matrix). */
+void oracle_evaporate_levels(OracleColony *o, uint8_t delta, uint8_t q_min);
 void oracle_deposit_iter(OracleColony *o);
 void oracle_deposit_best(OracleColony *o);
 void oracle_epochs(OracleColony *o, int count, int nthreads);

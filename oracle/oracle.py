@@ -51,6 +51,9 @@ _SIG = {
     "oracle_set_ant_results": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "oracle_evaporate": (None, [C.c_void_p]),
     "oracle_deposit_tour": (None, [C.c_void_p, i32p, C.c_int, C.c_float]),
+    "oracle_deposit_routes": (None, [C.c_void_p, i32p, i32p, C.c_int, C.c_int, C.c_longlong,
+                                     C.c_int, C.c_float, C.c_int]),
+    "oracle_evaporate_levels": (None, [C.c_void_p, C.c_uint8, C.c_uint8]),
     "oracle_deposit_iter": (None, [C.c_void_p]),
     "oracle_deposit_best": (None, [C.c_void_p]),
     "oracle_epochs": (None, [C.c_void_p, C.c_int, C.c_int]),
@@ -209,6 +212,26 @@ class Oracle:
     def deposit_tour(self, tour, delta):
         t = np.ascontiguousarray(tour, np.int32)
         self.lib.oracle_deposit_tour(self.h, _p(t, C.c_int32), len(t), delta)
+
+    def deposit_solution(self, routes, lens, K, m, ant, delta):
+        """kernel_deposit_solution's layout: routes[steps][m], lens[m][K]."""
+        r = np.ascontiguousarray(routes, np.int32)
+        ln = np.ascontiguousarray(lens, np.int32)
+        assert r.ndim == 2 and r.shape[1] == m and ln.shape == (m, K)
+        self.lib.oracle_deposit_routes(self.h, _p(r, C.c_int32), _p(ln, C.c_int32), K, m,
+                                       r.shape[0], ant, delta, 1)
+
+    def deposit_flat_solution(self, routes, lens, delta):
+        """kernel_deposit_flat_solution's layout: routes[K][n+1], lens[K]."""
+        r = np.ascontiguousarray(routes, np.int32)
+        ln = np.ascontiguousarray(lens, np.int32)
+        K = ln.shape[0]
+        assert r.shape == (K, self.n + 1)
+        self.lib.oracle_deposit_routes(self.h, _p(r, C.c_int32), _p(ln, C.c_int32), K, 1, 0, 0,
+                                       delta, 0)
+
+    def evaporate_levels(self, delta, q_min):
+        self.lib.oracle_evaporate_levels(self.h, delta, q_min)
 
     def deposit_iter(self):
         self.lib.oracle_deposit_iter(self.h)

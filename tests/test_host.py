@@ -22,17 +22,80 @@ def _declared(header):
                   {"if", "for", "while", "return", "sizeof", "defined"})
 
 
-@pytest.mark.parametrize("header", ["aco_hip.h", "aco_host.h"])
+@pytest.mark.parametrize("header", ["aco_hip.h", "aco_hip_pp.h", "aco_host.h"])
 def test_library_exports_declared_symbols(header):
     lib = C.CDLL(N.LIB_PATH)
     names = _declared(header)
-    assert len(names) > 10
+    assert len(names) >= 10
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 
 def test_binding_table_covers_c_abi():
     assert set(_declared("aco_hip.h")) <= set(N.HIP_SYMBOLS)
+    assert set(_declared("aco_hip_pp.h")) == set(N.PP_SYMBOLS)
+
+
+def test_plain_pointer_calls_check_host_arguments():
+    """Bad sizes and null buffers are rejected before anything is enqueued
+    (no device is touched, so this runs without a GPU)."""
+    lib = N.load()
+    bad = 1  # ACO_HIP_ERR_INVALID_INPUT
+    fake = C.c_void_p(4096)  # never dereferenced: every call below fails its checks
+    assert lib.aco_hip_pp_init_tau(None, 10, 170, None) == bad
+    assert lib.aco_hip_pp_init_tau(fake, 0, 170, None) == bad
+    assert lib.aco_hip_pp_init_tau(fake, 65535, 170, None) == bad
+    assert b"n must be" in lib.aco_hip_pp_last_error()
+    assert lib.aco_hip_pp_evaporate_tau(None, 10, 13, 0, None) == bad
+    assert lib.aco_hip_pp_build_candidate_lists(fake, fake, 10, 0, 0, 0, fake, fake, None) == bad
+    assert lib.aco_hip_pp_build_candidate_lists(fake, fake, 10, 513, 0, 0, fake, fake,
+                                                None) == bad
+    assert lib.aco_hip_pp_build_candidate_lists(fake, fake, 10, 8, 2, 0, fake, fake, None) == bad
+    assert lib.aco_hip_pp_build_candidate_lists(fake, fake, 10, 8, 0, 2, fake, fake, None) == bad
+    assert lib.aco_hip_pp_build_candidate_lists(fake, None, 10, 8, 0, 0, fake, fake, None) == bad
+    assert lib.aco_hip_pp_deposit_solution(fake, 10, 2, 4, 30, fake, fake, 0.1, 4, None) == bad
+    assert lib.aco_hip_pp_deposit_solution(fake, 10, 2, 4, 30, fake, fake, 0.1, -1, None) == bad
+    assert lib.aco_hip_pp_deposit_solution(fake, 10, 0, 4, 30, fake, fake, 0.1, 0, None) == bad
+    assert lib.aco_hip_pp_deposit_solution(fake, 10, 2, 4, 0, fake, fake, 0.1, 0, None) == bad
+    assert lib.aco_hip_pp_deposit_flat_solution(fake, 10, 0, fake, fake, 0.1, None) == bad
+    assert lib.aco_hip_pp_deposit_flat_solution(fake, 10, 2, None, fake, 0.1, None) == bad
+    assert lib.aco_hip_pp_deposit_tour(fake, 10, fake, -1, 0.1, None) == bad
+    assert lib.aco_hip_pp_deposit_tour(fake, 10, None, 5, 0.1, None) == bad
+    assert lib.aco_hip_pp_deposit_tour(fake, 10, None, 1, 0.1, None) == 0  # no edge: no-op
+    assert lib.aco_hip_pp_route_costs(fake, fake, 10, fake, fake, -1, 5, 0, fake, None) == bad
+    assert lib.aco_hip_pp_route_costs(fake, fake, 10, fake, fake, 3, 0, 0, fake, None) == bad
+    assert lib.aco_hip_pp_route_costs(fake, fake, 10, fake, fake, 3, 5, 3, fake, None) == bad
+    assert lib.aco_hip_pp_route_costs(None, fake, 10, fake, fake, 3, 5, 0, fake, None) == bad
+    assert lib.aco_hip_pp_route_costs(None, None, 10, None, None, 0, 5, 0, None, None) == 0
+
+
+@pytest.mark.parametrize("dist_mode", [0, 1])
+def test_plain_pointer_eta_table(dist_mode):
+    """aco_hip_pp_eta_table covers every distance of the coordinate range and
+    holds the context's heuristic values (the table builders of
+    csrc/host/aco_tables.c, pinned against the oracle's in test_oracle)."""
+    lib = N.load()
+    ln = C.c_int32()
+    assert lib.aco_hip_pp_eta_table(dist_mode, 2.0, None, C.byref(ln)) == 0
+    eta = np.zeros(ln.value, np.float32)
+    assert lib.aco_hip_pp_eta_table(dist_mode, 2.0, N.ptr(eta, C.c_float), C.byref(ln)) == 0
+    if dist_mode == 0:
+        # the longest edge of [0, 32767]^2 is nint(32767 * sqrt(2)) = 46340
+        assert ln.value == 46341
+        d = np.arange(1, ln.value, dtype=np.float64)
+        assert np.array_equal(eta[1:], (d ** -2.0).astype(np.float32))
+    else:
+        # bins of relative width 2^-10 from d = 1: the last one holds the
+        # longest edge, float(sqrt(2 * 32767^2))
+        top = np.float32(np.sqrt(np.float64(2 * 32767 ** 2)))
+        last = 1 + ((int(top.view(np.uint32)) - 0x3F800000) >> 13)
+        assert ln.value == last + 1
+        b = np.arange(1, ln.value, dtype=np.uint32)
+        centre = (np.uint32(0x3F800000) + ((b - 1) << 13) + np.uint32(4096)).view(np.float32)
+        assert np.all(np.abs(eta[1:].astype(np.float64) * centre.astype(np.float64) ** 2 - 1)
+                      < 2 ** -23)
+    assert eta[0] == np.float32(np.float64(np.float32(1e-9)) ** -2.0)
+    assert np.all(np.diff(eta[1:]) < 0)
 
 
 def test_reader_reference_instances():

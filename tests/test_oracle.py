@@ -265,6 +265,119 @@ def test_deposit_sequential_semantics():
     assert np.array_equal(o.tau()[1], np.take_along_axis(o.tau()[0], idx, axis=1))
 
 
+def test_evaporate_levels_closed_form():
+    """kernel_evaporate_tau with any delta and q_min (the saturation in int:
+    q_min + delta may exceed 255): equals max(q - delta, q_min) clamped at
+    0, over every level and a spread of (delta, q_min)."""
+    inst = Instance.uniform(15, 2, seed=5)
+    o = O.Oracle(inst, cand_k=4, ants=4)
+    q = np.resize(np.arange(256, dtype=np.uint8), (16, 16))
+    for delta, q_min in [(13, 0), (2, 0), (1, 7), (40, 200), (255, 255), (0, 0), (200, 100),
+                         (0, 255)]:
+        o.set_tau(q)
+        o.evaporate_levels(delta, q_min)
+        want = np.maximum(np.maximum(q.astype(np.int64) - delta, 0), q_min).astype(np.uint8)
+        assert np.array_equal(o.tau()[0], want), (delta, q_min)
+
+
+def reference_layouts(tour, n, K, m, ant, steps):
+    """The reference's two route layouts of one giant tour: the colony's
+    step-major routes[steps][m] / lens[m][K] as kernel_construct_solutions
+    writes them (aco_cuda_kernels.cu:520-548: step 0 is the depot, each
+    route's nodes follow, its closing depot is the next route's first), and
+    the flat routes[K][n+1] / lens[K] of aco_cuda.cu:373-379 (each route's
+    nodes with both depots)."""
+    routes, cur = [], [0]
+    for v in tour[1:]:
+        cur.append(int(v))
+        if v == 0:
+            routes.append(cur)
+            cur = [0]
+    assert len(routes) == K
+    step = np.full((steps, m), -7, np.int32)
+    slen = np.zeros((m, K), np.int32)
+    g = 0
+    step[0, ant] = 0
+    for v, r in enumerate(routes):
+        for node in r[1:]:
+            g += 1
+            step[g, ant] = node
+        slen[ant, v] = len(r)
+    flat = np.full((K, n + 1), -7, np.int32)
+    flen = np.zeros(K, np.int32)
+    for v, r in enumerate(routes):
+        flat[v, :len(r)] = r
+        flen[v] = len(r)
+    return step, slen, flat, flen
+
+
+@pytest.mark.parametrize("tour", [
+    [0, 4, 0, 0, 9, 3, 0, 0, 0],            # single-customer and empty routes
+    [0, 0, 0, 0, 0],                         # all routes empty
+    [0, 5, 6, 7, 0, 1, 2, 0, 8, 0, 3, 0],   # ordinary routes
+])
+def test_route_layout_deposits_equal_tour_deposit(tour):
+    """Per-route deposits read in the reference's step-major and flat layouts
+    equal the deposit of the same routes as one giant tour (the layouts are
+    built independently, by replaying the reference's writers)."""
+    inst = Instance.uniform(30, 4, seed=6)
+    K = tour.count(0) - 1
+    inst = Instance(inst.x, inst.y, inst.demand, K, 30)
+    m, ant = 5, 3
+    step, slen, flat, flen = reference_layouts(tour, 30, K, m, ant, 40)
+    rng = np.random.default_rng(11)
+    tau = rng.integers(0, 256, size=(31, 31), dtype=np.uint8)
+    want = O.Oracle(inst, cand_k=8, ants=4)
+    want.set_tau(tau)
+    want.deposit_tour(tour, 0.7)
+    for kind in ("solution", "flat"):
+        o = O.Oracle(inst, cand_k=8, ants=4)
+        o.set_tau(tau)
+        if kind == "solution":
+            o.deposit_solution(step, slen, K, m, ant, 0.7)
+        else:
+            o.deposit_flat_solution(flat, flen, 0.7)
+        assert np.array_equal(o.tau()[0], want.tau()[0]), kind
+        assert np.array_equal(o.tau()[1], want.tau()[1]), kind
+    assert not np.array_equal(want.tau()[0], tau)
+
+
+def test_route_layout_deposits_skip_bad_entries():
+    """A node outside 0..n, or a route running past the buffer, deposits
+    nothing on the edges that touch it; the other edges are unchanged."""
+    inst = Instance.uniform(30, 3, seed=6)
+    tour = [0, 5, 6, 7, 0, 1, 2, 0, 8, 0]
+    m, ant = 2, 0
+    step, slen, flat, flen = reference_layouts(tour, 30, 3, m, ant, 12)
+    rng = np.random.default_rng(3)
+    tau = rng.integers(0, 256, size=(31, 31), dtype=np.uint8)
+    # node 6 of route 0 becomes 99: edges (5,99), (99,7) deposit nothing
+    step[2, ant] = 99
+    flat[0, 2] = 99
+    # route 2 runs 3 steps past the end of the step buffer / is longer than its row
+    slen[ant, 2] = 6
+    flen[2] = 33
+    flat[2, 3:] = 0
+    want = O.Oracle(inst, cand_k=8, ants=4)
+    want.set_tau(tau)
+    want.deposit_tour([0, 5], 0.7)
+    want.deposit_tour([7, 0, 1, 2, 0, 8, 0], 0.7)
+    o = O.Oracle(inst, cand_k=8, ants=4)
+    o.set_tau(tau)
+    o.deposit_solution(step, slen, 3, m, ant, 0.7)
+    assert np.array_equal(o.tau()[0], want.tau()[0])
+    # flat: route 2 is 0 8 0 then zeros to the end of its row: 0->0 edges
+    # (29 of them) that the 33-node length runs past the last row do not add
+    want_f = O.Oracle(inst, cand_k=8, ants=4)
+    want_f.set_tau(tau)
+    want_f.deposit_tour([0, 5], 0.7)
+    want_f.deposit_tour([7, 0, 1, 2, 0, 8] + [0] * 29, 0.7)
+    o = O.Oracle(inst, cand_k=8, ants=4)
+    o.set_tau(tau)
+    o.deposit_flat_solution(flat, flen, 0.7)
+    assert np.array_equal(o.tau()[0], want_f.tau()[0])
+
+
 def test_epoch_loop_improves_and_stops():
     inst = Instance.uniform(100, 8, seed=1234)
     o = O.Oracle(inst, cand_k=15, ants=64, rho=0.1, seed=1234)

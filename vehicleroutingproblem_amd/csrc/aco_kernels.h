@@ -148,16 +148,21 @@ static inline __device__ uint32_t edge_code_t(int2 a, int2 b) {
   const uint32_t s = edge_sq(a, b);
   return REAL ? aco_real_eta_bin(aco_real_dist_s(s)) : (uint32_t)aco_nint_sqrt_u32(s);
 }
+static inline __device__ uint32_t edge_code_r(int real, int2 a, int2 b) {
+  return real ? edge_code_t<true>(a, b) : edge_code_t<false>(a, b);
+}
 static inline __device__ uint32_t edge_code(const DevCtx &P, int2 a, int2 b) {
-  return P.dist_real ? edge_code_t<true>(a, b) : edge_code_t<false>(a, b);
+  return edge_code_r(P.dist_real, a, b);
 }
 
 /* Cost of an edge in cost units: the nint distance, or in real mode the
  * float distance times 2^23 (exact). */
-static inline __device__ long long edge_units(const DevCtx &P, int2 a, int2 b) {
+static inline __device__ long long edge_units_r(int real, int2 a, int2 b) {
   const uint32_t s = edge_sq(a, b);
-  return P.dist_real ? (long long)aco_real_units(aco_real_dist_s(s))
-                     : (long long)aco_nint_sqrt_u32(s);
+  return real ? (long long)aco_real_units(aco_real_dist_s(s)) : (long long)aco_nint_sqrt_u32(s);
+}
+static inline __device__ long long edge_units(const DevCtx &P, int2 a, int2 b) {
+  return edge_units_r(P.dist_real, a, b);
 }
 
 /* Tour length L of a cost in cost units, as the float the deposits divide
@@ -185,6 +190,26 @@ static inline __host__ __device__ int aco_significant(double prev, double now, d
 /* kernels (aco_kernels.hip; construction: aco_construct.h, instantiated in
  * aco_construct_fsm.hip for FSM = true and aco_construct_tab.hip for false) */
 __global__ void k_knn(DevCtx P, int total_rows, int knn_key, float depot_w);
+/* quantization tables passed by value (2 KB of kernel arguments) */
+struct PpTables {
+  float deq[256];
+  float thr[255];
+};
+/* kernels of the plain-pointer entry points (caller-owned buffers, the
+ * reference's pheromone layout: (n+1)^2 bytes, row stride n+1) */
+__global__ void k_pp_fill(uint8_t *tau, size_t bytes, int q);
+__global__ void k_pp_evaporate(uint8_t *tau, size_t bytes, int delta, int q_min);
+__global__ void k_pp_knn(const int32_t *x, const int32_t *y, int n, int k, int knn_key, int real,
+                         int32_t *cand, int32_t *code);
+__global__ void k_pp_deposit_tour(uint8_t *tau, int n, const int32_t *tour, int len, float delta,
+                                  PpTables tab);
+template <bool STEP>
+__global__ void k_pp_deposit_routes(uint8_t *tau, int n, int K, int m, long long steps,
+                                    const int32_t *routes, const int32_t *lens, int ant,
+                                    float delta, PpTables tab);
+__global__ void k_pp_route_costs(const int32_t *x, const int32_t *y, int n, const int32_t *tours,
+                                 const int32_t *lens, int count, int stride, int real,
+                                 long long *costs);
 __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
 template <bool STAGE, bool STATS, int ROWS, bool XCD, bool FSM>
 __global__ void k_construct(DevCtx P, int groups_per_colony, int W, const int *cols);

@@ -23,24 +23,16 @@
  * (aco_cuda_kernels.cu:164-214): customers only, self excluded, ties to the
  * lower index, slots past n-1 padded with index 0.  Each record carries the
  * slot's distance code (the eta-table index) in its upper 16 bits. */
-__global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_key,
-                                              float depot_w) {
-  const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= total_rows) return;
-  const int ii = P.row_inst[row];
-  const InstDesc I = P.inst[ii];
-  const int i = row - I.row_base;
-  const int n = I.n, k = I.k;
-  const int2 *xy = P.xy + I.node_off;
-  const int2 pi = xy[i];
+template <class XY, class Emit>
+__device__ __forceinline__ void knn_row(XY &&xy, int n, int i, int k, int knn_key, int real,
+                                        int lane, Emit &&emit) {
+  const int2 pi = xy(i);
   uint64_t last = 0;
-  uint32_t *rec = P.rec + I.cand_off + (size_t)i * k;
   for (int t = 0; t < k; ++t) {
     uint64_t best = ~0ull;
     for (int j = 1 + lane; j <= n; j += 64) {
       if (j == i) continue;
-      const int2 pj = xy[j];
+      const int2 pj = xy(j);
       const int dx = pi.x - pj.x, dy = pi.y - pj.y;
       const uint32_t s = (uint32_t)(dx * dx) + (uint32_t)(dy * dy);
       uint64_t hi;
@@ -55,19 +47,37 @@ __global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_k
     }
     best = wave_min_u64(best);
     if (lane == 0) {
-      uint32_t r = 0;
       if (best != ~0ull) {
         const int j = (int)(best & 0xFFFFFFu);
-        const uint32_t d = edge_code(P, pi, xy[j]);
-        r = (uint32_t)j | (d << 16);
+        emit(t, j, edge_code_r(real, pi, xy(j)));
+      } else {
+        emit(t, 0, 0u);
       }
-      rec[t] = r;
-      P.cidx[I.cand_off + (size_t)i * k + t] = (uint16_t)(r & 0xFFFFu);
     }
     last = best;
   }
+}
+
+__global__ void __launch_bounds__(256) k_knn(DevCtx P, int total_rows, int knn_key,
+                                              float depot_w) {
+  const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= total_rows) return;
+  const int ii = P.row_inst[row];
+  const InstDesc I = P.inst[ii];
+  const int i = row - I.row_base;
+  const int n = I.n, k = I.k;
+  const int2 *xy = P.xy + I.node_off;
+  uint32_t *rec = P.rec + I.cand_off + (size_t)i * k;
+  uint16_t *cidx = P.cidx + I.cand_off + (size_t)i * k;
+  knn_row([&](int j) { return xy[j]; }, n, i, k, knn_key, P.dist_real, lane,
+          [&](int t, int j, uint32_t d) {
+            const uint32_t r = (uint32_t)j | (d << 16);
+            rec[t] = r;
+            cidx[t] = (uint16_t)j;
+          });
   if (lane == 0) {
-    const float ds = depot_w * P.eta[edge_code(P, pi, xy[0])];
+    const float ds = depot_w * P.eta[edge_code(P, xy[i], xy[0])];
     P.dscore[I.node_off + i] = ds == ds ? ds : 0.0f; /* NaN scores never close a route */
   }
 }
@@ -595,26 +605,38 @@ __device__ __forceinline__ void patch_slot(const DevCtx &P, const UpdTables &T,
   }
 }
 
-/* Deposit one tour: per edge new = quantize(dequantize(tau[u][v]) + delta),
- * max into tau[u][v] and tau[v][u] (kernel_deposit_solution /
- * kernel_deposit_flat_solution, aco_cuda_kernels.cu:592-668).  One thread per
- * edge; the only entries two edges share are those of a single-customer
- * route 0-c-0 (one thread does both edges) and the (0,0) entry of empty
- * routes (one thread applies them all), so the result equals the sequential
- * edge order.  The candidate-aligned copy, its scores and the all-pairs
- * weights are patched alongside.  Two dependent memory round trips per edge.  Ends with a
- * workgroup barrier. */
-__device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDesc &D,
-                             const InstDesc &I, const int32_t *tour, int len, float delta) {
+/* Deposit one tour on the pheromone matrix tau (row stride S): per edge
+ * new = quantize(dequantize(tau[u][v]) + delta), max into tau[u][v] and
+ * tau[v][u] (kernel_deposit_solution / kernel_deposit_flat_solution,
+ * aco_cuda_kernels.cu:592-668).  One thread per edge; the only entries two
+ * edges share are those of a single-customer route 0-c-0 (one thread does
+ * both edges) and the (0,0) entry of empty routes (one thread applies them
+ * all), so the result equals the sequential edge order when every customer
+ * appears at most once.  Edges with a node outside 0..n1-1 deposit nothing.
+ * commit(u, v, tuv, tvu) stores an edge's two new levels (u, v != both 0),
+ * commit00(x) the (0,0) level.  Ends with a workgroup barrier. */
+__device__ __forceinline__ void dep_edge(const uint8_t *__restrict__ tau, int S, int u, int v,
+                                         bool both, float delta, const float *deq,
+                                         const float *thr, uint32_t &tuv, uint32_t &tvu) {
+  const uint32_t a = tau[(size_t)u * S + v];
+  const uint32_t b = tau[(size_t)v * S + u];
+  uint32_t nq = aco_quantize_thr(thr, deq[a] + delta);
+  tuv = nq > a ? nq : a;
+  tvu = nq > b ? nq : b;
+  if (both) {
+    /* single-customer route 0 -> v -> 0: the second edge (v, 0) in order */
+    nq = aco_quantize_thr(thr, deq[tvu] + delta);
+    tvu = nq > tvu ? nq : tvu;
+    tuv = nq > tuv ? nq : tuv;
+  }
+}
+
+template <class Commit, class Commit00>
+__device__ __forceinline__ void deposit_core(const uint8_t *__restrict__ tau, int S, int n1,
+                                             const int32_t *tour, int len, float delta,
+                                             const float *deq, const float *thr,
+                                             Commit &&commit, Commit00 &&commit00) {
   __shared__ int s_zz_count, s_zz_first;
-  const int n1 = I.n + 1;
-  uint8_t *__restrict__ tau = P.pher + D.pher_off;
-  uint8_t *__restrict__ tc = P.pher + D.tcand_off;
-  float *__restrict__ score = P.score + D.score_off;
-  const uint8_t *__restrict__ so = P.slot_of + I.slot_off;
-  const int2 *__restrict__ xy = P.xy + I.node_off;
-  float *__restrict__ fs = P.fscore ? P.fscore + D.fs_off : nullptr;
-  const int S = I.fs_stride;
   if (threadIdx.x == 0) {
     s_zz_count = 0;
     s_zz_first = 0x7FFFFFFF;
@@ -635,45 +657,55 @@ __device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDe
       if (p == s_zz_first) {
         uint32_t x = tau[0];
         for (int r = 0; r < s_zz_count; ++r) {
-          const uint32_t nq = dep_new(T, x, delta);
+          const uint32_t nq = aco_quantize_thr(thr, deq[x] + delta);
           x = nq > x ? nq : x;
         }
-        tau[0] = (uint8_t)x;
-        patch_slot(P, T, I, tc, score, 0, 0, so[0], 0, (uint8_t)x);
-        if (fs) fs[0] = T.taupow[x] * T.eta[0];
+        commit00(x);
       }
       continue;
     }
+    if ((unsigned)u >= (unsigned)n1 || (unsigned)v >= (unsigned)n1) continue;
     if (u != 0 && v == 0 && um1 == 0) continue; /* done by thread p-1 */
-    const uint32_t a = tau[(size_t)u * S + v];
-    const uint32_t b = tau[(size_t)v * S + u];
-    const int su = so[(size_t)u * n1 + v];
-    const int sv = so[(size_t)v * n1 + u];
-    const int d = (int)edge_code(P, xy[u], xy[v]);
     uint32_t tuv, tvu;
-    if (u == 0 && vp1 == 0) {
-      /* single-customer route 0 -> v -> 0: both edges in order */
-      uint32_t nq = dep_new(T, a, delta);
-      tuv = nq > a ? nq : a;
-      tvu = nq > b ? nq : b;
-      nq = dep_new(T, tvu, delta);
-      tvu = nq > tvu ? nq : tvu;
-      tuv = nq > tuv ? nq : tuv;
-    } else {
-      const uint32_t nq = dep_new(T, a, delta);
-      tuv = nq > a ? nq : a;
-      tvu = nq > b ? nq : b;
-    }
-    tau[(size_t)u * S + v] = (uint8_t)tuv;
-    tau[(size_t)v * S + u] = (uint8_t)tvu;
-    patch_slot(P, T, I, tc, score, u, v, su, d, (uint8_t)tuv);
-    patch_slot(P, T, I, tc, score, v, u, sv, d, (uint8_t)tvu);
-    if (fs) {
-      fs[(size_t)u * S + v] = T.taupow[tuv] * T.eta[d];
-      fs[(size_t)v * S + u] = T.taupow[tvu] * T.eta[d];
-    }
+    dep_edge(tau, S, u, v, u == 0 && vp1 == 0, delta, deq, thr, tuv, tvu);
+    commit(u, v, tuv, tvu);
   }
   __syncthreads();
+}
+
+/* deposit_core on a colony of the context: the candidate-aligned copy, its
+ * scores and the all-pairs weights are patched alongside.  Two dependent
+ * memory round trips per edge. */
+__device__ void deposit_tour(const DevCtx &P, const UpdTables &T, const ColonyDesc &D,
+                             const InstDesc &I, const int32_t *tour, int len, float delta) {
+  const int n1 = I.n + 1;
+  uint8_t *__restrict__ tau = P.pher + D.pher_off;
+  uint8_t *__restrict__ tc = P.pher + D.tcand_off;
+  float *__restrict__ score = P.score + D.score_off;
+  const uint8_t *__restrict__ so = P.slot_of + I.slot_off;
+  const int2 *__restrict__ xy = P.xy + I.node_off;
+  float *__restrict__ fs = P.fscore ? P.fscore + D.fs_off : nullptr;
+  const int S = I.fs_stride;
+  deposit_core(
+      tau, S, n1, tour, len, delta, T.deq, T.thr,
+      [&](int u, int v, uint32_t tuv, uint32_t tvu) {
+        const int su = so[(size_t)u * n1 + v];
+        const int sv = so[(size_t)v * n1 + u];
+        const int d = (int)edge_code(P, xy[u], xy[v]);
+        tau[(size_t)u * S + v] = (uint8_t)tuv;
+        tau[(size_t)v * S + u] = (uint8_t)tvu;
+        patch_slot(P, T, I, tc, score, u, v, su, d, (uint8_t)tuv);
+        patch_slot(P, T, I, tc, score, v, u, sv, d, (uint8_t)tvu);
+        if (fs) {
+          fs[(size_t)u * S + v] = T.taupow[tuv] * T.eta[d];
+          fs[(size_t)v * S + u] = T.taupow[tvu] * T.eta[d];
+        }
+      },
+      [&](uint32_t x) {
+        tau[0] = (uint8_t)x;
+        patch_slot(P, T, I, tc, score, 0, 0, so[0], 0, (uint8_t)x);
+        if (fs) fs[0] = T.taupow[x] * T.eta[0];
+      });
 }
 
 __device__ __forceinline__ UpdTables stage_tables(const DevCtx &P) {
@@ -1067,4 +1099,211 @@ __global__ void __launch_bounds__(256) k_route_costs(DevCtx P, int instance, con
   }
   acc = wave_sum_i64(acc);
   if (lane == 0) out[t] = acc;
+}
+
+/* ---- plain-pointer kernels ------------------------------------------------ */
+
+/* f applied to every byte of p[0, bytes): the unaligned head and tail byte by
+ * byte, the rest as 16-byte vectors (one grid-stride pass). */
+template <class F>
+__device__ __forceinline__ void bytes_map(uint8_t *p, size_t bytes, F &&f) {
+  const size_t head = ((16 - ((uintptr_t)p & 15)) & 15) < bytes ? ((16 - ((uintptr_t)p & 15)) & 15)
+                                                               : bytes;
+  const size_t nvec = (bytes - head) / 16;
+  const size_t tail0 = head + nvec * 16;
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t nth = (size_t)gridDim.x * blockDim.x;
+  if (tid < head) p[tid] = (uint8_t)f((uint32_t)p[tid]);
+  if (tid < bytes - tail0) p[tail0 + tid] = (uint8_t)f((uint32_t)p[tail0 + tid]);
+  uint4 *body = reinterpret_cast<uint4 *>(p + head);
+  for (size_t e = tid; e < nvec; e += nth) {
+    uint4 v = body[e];
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      uint32_t r = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) r |= f((w[i] >> (8 * b)) & 0xFFu) << (8 * b);
+      w[i] = r;
+    }
+    body[e] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+/* kernel_init_tau (aco_cuda_kernels.cu:131-136): every byte = q */
+__global__ void __launch_bounds__(256) k_pp_fill(uint8_t *tau, size_t bytes, int q) {
+  bytes_map(tau, bytes, [=](uint32_t) { return (uint32_t)q; });
+}
+
+/* kernel_evaporate_tau (aco_cuda_kernels.cu:145-155): q > q_min + delta ?
+ * q - delta : q_min, with q_min + delta in int (no uint8 wrap). */
+__global__ void __launch_bounds__(256) k_pp_evaporate(uint8_t *tau, size_t bytes, int delta,
+                                                      int q_min) {
+  bytes_map(tau, bytes, [=](uint32_t q) {
+    return (int)q > q_min + delta ? q - (uint32_t)delta : (uint32_t)q_min;
+  });
+}
+
+/* Candidate lists on caller buffers: row i of cand / code (stride k) holds
+ * the k nearest customers of node i and their heuristic-table index, in the
+ * order of k_knn (one wavefront per row). */
+__global__ void __launch_bounds__(256) k_pp_knn(const int32_t *x, const int32_t *y, int n, int k,
+                                                int knn_key, int real, int32_t *cand,
+                                                int32_t *code) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i > n) return;
+  int32_t *cr = cand + (size_t)i * k;
+  int32_t *dr = code + (size_t)i * k;
+  knn_row([&](int j) { return make_int2(x[j], y[j]); }, n, i, k, knn_key, real, lane,
+          [&](int t, int j, uint32_t d) {
+            cr[t] = j;
+            dr[t] = (int32_t)d;
+          });
+}
+
+__device__ __forceinline__ void pp_stage_tables(const PpTables &tab, float *s_deq, float *s_thr) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    s_deq[i] = tab.deq[i];
+    s_thr[i] = i < 255 ? tab.thr[i] : 0.0f;
+  }
+  __syncthreads();
+}
+
+/* deposit_core on a caller-owned matrix (row stride n+1), one workgroup. */
+__global__ void __launch_bounds__(1024) k_pp_deposit_tour(uint8_t *tau, int n,
+                                                          const int32_t *tour, int len,
+                                                          float delta, PpTables tab) {
+  __shared__ float s_deq[256], s_thr[256];
+  pp_stage_tables(tab, s_deq, s_thr);
+  const int S = n + 1;
+  deposit_core(
+      tau, S, S, tour, len, delta, s_deq, s_thr,
+      [&](int u, int v, uint32_t tuv, uint32_t tvu) {
+        tau[(size_t)u * S + v] = (uint8_t)tuv;
+        tau[(size_t)v * S + u] = (uint8_t)tvu;
+      },
+      [&](uint32_t xq) { tau[0] = (uint8_t)xq; });
+}
+
+/* The reference's per-route deposits on caller buffers, one workgroup:
+ * kernel_deposit_solution (aco_cuda_kernels.cu:592-627, STEP: node pos of
+ * route v is routes[(off_v + pos) * m + ant] with off_v the sum of
+ * lens[ant*K + u] - 1 over u < v) and kernel_deposit_flat_solution (:637-668,
+ * !STEP: node pos of route v is routes[v * (n+1) + pos]).  Each route starts
+ * from the depot whatever its entry 0 holds, and deposits its edges
+ * pos-1 -> pos for pos in 1..len-1.  Entries outside the buffer (steps rows of
+ * m, or K rows of n+1) and nodes outside 0..n deposit nothing.  Conflicts as
+ * deposit_core: a route's 0 -> c -> 0 is one lane's, and the (0,0) entry of
+ * every 0 -> 0 edge is applied by one thread at the end. */
+template <bool STEP>
+__global__ void __launch_bounds__(1024) k_pp_deposit_routes(uint8_t *tau, int n, int K, int m,
+                                                            long long steps,
+                                                            const int32_t *routes,
+                                                            const int32_t *lens, int ant,
+                                                            float delta, PpTables tab) {
+  __shared__ float s_deq[256], s_thr[256];
+  __shared__ long long s_scan[1024];
+  __shared__ int s_len[1024];
+  __shared__ int s_zz;
+  __shared__ long long s_carry;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwaves = nt >> 6;
+  const int S = n + 1;
+  if (tid == 0) {
+    s_zz = 0;
+    s_carry = 0;
+  }
+  pp_stage_tables(tab, s_deq, s_thr);
+  const long long limit = STEP ? steps : (long long)K * S;
+  for (int v0 = 0; v0 < K; v0 += nt) {
+    const int cnt = K - v0 < nt ? K - v0 : nt;
+    const int len = tid < cnt ? lens[STEP ? (size_t)ant * K + v0 + tid : (size_t)v0 + tid] : 1;
+    s_len[tid] = len;
+    /* inclusive scan of len - 1 (Hillis-Steele over the workgroup) */
+    s_scan[tid] = (long long)len - 1;
+    __syncthreads();
+    for (int d = 1; d < nt; d <<= 1) {
+      const long long add = tid >= d ? s_scan[tid - d] : 0;
+      __syncthreads();
+      s_scan[tid] += add;
+      __syncthreads();
+    }
+    const long long carry = s_carry;
+    for (int w = wave; w < cnt; w += nwaves) {
+      const int L = s_len[w];
+      const long long base =
+          STEP ? carry + s_scan[w] - ((long long)s_len[w] - 1) : (long long)(v0 + w) * S;
+      /* node at position i of the route, the depot at 0 */
+      auto seq = [&](int i) -> int {
+        if (i == 0) return 0;
+        const long long e = base + i;
+        if (e < 0 || e >= limit) return -1;
+        return STEP ? routes[(size_t)e * m + ant] : routes[(size_t)e];
+      };
+      for (int pos = 1 + lane; pos < L; pos += 64) {
+        const int u = seq(pos - 1), v = seq(pos);
+        if ((unsigned)u > (unsigned)n || (unsigned)v > (unsigned)n) continue;
+        if (u == 0 && v == 0) {
+          atomicAdd(&s_zz, 1);
+          continue;
+        }
+        const int um1 = pos >= 2 ? seq(pos - 2) : -1;
+        const int vp1 = pos + 1 < L ? seq(pos + 1) : -1;
+        if (u != 0 && v == 0 && um1 == 0) continue; /* done by the lane at pos-1 */
+        uint32_t tuv, tvu;
+        dep_edge(tau, S, u, v, u == 0 && vp1 == 0, delta, s_deq, s_thr, tuv, tvu);
+        tau[(size_t)u * S + v] = (uint8_t)tuv;
+        tau[(size_t)v * S + u] = (uint8_t)tvu;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) s_carry = carry + s_scan[cnt - 1];
+    __syncthreads();
+  }
+  if (tid == 0 && s_zz > 0) {
+    uint32_t xq = tau[0];
+    for (int r = 0; r < s_zz; ++r) {
+      const uint32_t nq = aco_quantize_thr(s_thr, s_deq[xq] + delta);
+      xq = nq > xq ? nq : xq;
+    }
+    tau[0] = (uint8_t)xq;
+  }
+}
+template __global__ void k_pp_deposit_routes<true>(uint8_t *, int, int, int, long long,
+                                                   const int32_t *, const int32_t *, int, float,
+                                                   PpTables);
+template __global__ void k_pp_deposit_routes<false>(uint8_t *, int, int, int, long long,
+                                                    const int32_t *, const int32_t *, int, float,
+                                                    PpTables);
+
+/* k_route_costs on caller buffers: a tour with a length outside 0..stride or
+ * a node outside 0..n gets cost -1. */
+__global__ void __launch_bounds__(256) k_pp_route_costs(const int32_t *x, const int32_t *y, int n,
+                                                        const int32_t *tours,
+                                                        const int32_t *lens, int count,
+                                                        int stride, int real, long long *costs) {
+  const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= count) return;
+  const int32_t *tour = tours + (size_t)t * stride;
+  const int len = lens[t];
+  if (len < 0 || len > stride) {
+    if (lane == 0) costs[t] = -1;
+    return;
+  }
+  long long acc = 0;
+  bool bad = false;
+  for (int p = lane; p + 1 < len; p += 64) {
+    const int a = tour[p], b = tour[p + 1];
+    if ((unsigned)a > (unsigned)n || (unsigned)b > (unsigned)n) {
+      bad = true;
+      continue;
+    }
+    acc += edge_units_r(real, make_int2(x[a], y[a]), make_int2(x[b], y[b]));
+  }
+  if (len == 1 && lane == 0 && (unsigned)tour[0] > (unsigned)n) bad = true;
+  acc = wave_sum_i64(acc);
+  bad = ballot64(bad) != 0;
+  if (lane == 0) costs[t] = bad ? -1 : acc;
 }
