@@ -676,8 +676,12 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     const int pa = (int)__builtin_amdgcn_ubfe(sela, hs, 16u);
     uint32_t pk = (uint32_t)__builtin_amdgcn_ds_bpermute(
         pa, (int)__builtin_amdgcn_perm(st, j, 0x05040100u)); /* j | st << 16 */
-    /* an ant with no feasible candidate takes the whole-wave fallback */
-    if (__builtin_expect((e0 == 0u && ka0 != 0u) || (e1 == 0u && ka1 != 0u), 0)) {
+    /* an ant with no feasible candidate takes the whole-wave fallback.  The
+     * weights are >= 0, so a half's T (the sum of its candidates' weights) is
+     * > 0 exactly when one of them is > 0: one compare over the wave finds
+     * every constructing half with none (or a NaN sum, which the per-half
+     * test below settles as before) */
+    if (__builtin_expect((ballot64(!(T > 0.0f)) & alive_m) != 0ull, 0)) {
 #pragma unroll 1
       for (int q = 0; q < 2; ++q) {
         if ((q ? e1 : e0) != 0u || (q ? ka1 : ka0) == 0u) continue;
