@@ -581,6 +581,10 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   const size_t tour_elems = round16((size_t)D.stride * 2) / 2;
   uint16_t *const s_tour0 =
       RING ? nullptr : reinterpret_cast<uint16_t *>(smem + off) + (size_t)wave * 2 * tour_elems;
+  if (!RING) off += (size_t)W * 2 * tour_elems * 2;
+  /* 64 u16 per wave that the lanes with no entry to write store to, so the
+   * per-decision stores need no exec mask */
+  uint16_t *const s_sink = reinterpret_cast<uint16_t *>(smem + off) + (size_t)wave * 64;
   __syncthreads();
   if (STATS && threadIdx.x == 0)
     atomicAdd(P.stats + (size_t)colony * ACO_STATS + 5, clock64() - t_kernel);
@@ -631,8 +635,12 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
    * (aco_cuda_kernels.cu:501-520) */
   int cur = 0, room = Q, pos = 0, veh = 0, lim = n;
   int slack = (K - 1) * Q - (int)I.total_demand;
-  /* lanes 0 and 32 store their constructing ant's state and tour entries */
-  uint64_t wr_m = has1 ? 0x100000001ull : 1ull;
+  /* lanes 0 and 32 store their constructing ant's state and tour entries,
+   * the others (and those of a finished ant) to their sink */
+  const bool writer = t == 0 && (h == 0 || has1);
+  int wscale = writer ? 1 : 0;
+  uint16_t *st_w = writer ? st_h : s_sink + lane;
+  uint16_t *tr_w = writer && !RING ? tr_h + 1 : s_sink + lane;
 
   const unsigned long long t_start = STATS ? clock64() : 0ull;
   unsigned long long fb_n = 0, fb_cyc = 0, t_done = 0;
@@ -716,10 +724,8 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     const int md = (int)(pk >> 16);
     const uint64_t mvm = ballot64(mvw != 0u) & alive_m;
     const uint32_t st2 = st_h[j2]; /* before state[move] changes: j2 != move */
-    if (__builtin_amdgcn_inverse_ballot_w64(wr_m)) {
-      st_h[mvw] = (uint16_t)VISITED;
-      if constexpr (!RING) tr_h[pos + 1] = (uint16_t)mvw;
-    }
+    st_w[mvw * wscale] = (uint16_t)VISITED;
+    if constexpr (!RING) tr_w[pos * wscale] = (uint16_t)mvw;
     const int wl = (pos + 1) & 31; /* chunk slot of this decision's tour entry */
     if constexpr (RING)
       if (__builtin_amdgcn_inverse_ballot_w64(alive_m) && t == wl) ring = (int)mvw;
@@ -777,7 +783,11 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
         }
       }
       alive_m &= ~ballot64(veh >= K);
-      wr_m &= alive_m;
+      if (veh >= K) { /* a finished ant's writer lane stores to its sink */
+        wscale = 0;
+        st_w = s_sink + lane;
+        tr_w = s_sink + lane;
+      }
       ka0 = (uint32_t)alive_m & km32;
       ka1 = (uint32_t)(alive_m >> 32) & km32;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

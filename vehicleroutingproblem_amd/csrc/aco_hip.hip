@@ -96,14 +96,15 @@ static int fs_stride_of(int n1) { return (n1 + 15) & ~15; }
  * of k_construct and k_construct_pair): staged candidate rows (+64 padding
  * entries), depot scores and coordinates; the computed-fallback tables
  * (taupow, and eta unless eta_len == 0) when the weight matrix is off; per
- * wave a state array of S u16, a tour buffer of `stride` u16 and a 128-byte
- * store sink, or with `pair` two state arrays and (unless `ring`: the pair
- * then stores its tours in 128-byte chunks from registers) two tour buffers. */
+ * wave a state array of S u16 and a tour buffer of `stride` u16, or with
+ * `pair` two state arrays and (unless `ring`: the pair then stores its tours
+ * in 128-byte chunks from registers) two tour buffers, and a 128-byte store
+ * sink. */
 static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on, int stride,
                       int pair, int ring = 0) {
   const size_t state = (size_t)fs_stride_of(n1) * 2;
   const size_t tour = r16((size_t)stride * 2);
-  size_t b = (size_t)W * (pair ? 2 * state + (ring ? 0 : 2 * tour) : state + tour + 128);
+  size_t b = (size_t)W * ((pair ? 2 * state + (ring ? 0 : 2 * tour) : state + tour) + 128);
   if (!fs_on) b += 256 * 4 + r16((size_t)eta_len * 4);
   if (stage) {
     const size_t ent = (size_t)n1 * k + 64;
