@@ -618,9 +618,10 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   const uint32_t cid = (uint32_t)D.colony_in_inst;
   /* slots t < k of a half */
   const uint32_t km32 = k >= 32 ? 0xFFFFFFFFu : ((1u << k) - 1u);
+  const uint64_t km64 = (uint64_t)km32 | ((uint64_t)km32 << 32);
   /* lanes of the ants still constructing, and their candidate slots per half */
   uint64_t alive_m = has1 ? ~0ull : 0xFFFFFFFFull;
-  uint32_t ka0 = km32, ka1 = has1 ? km32 : 0u;
+  uint64_t ka = alive_m & km64;
   /* byte addresses for ds_bpermute: lane 32h, and this half's last candidate */
   int ub = (lane & 32) << 2;
   asm("" : "+v"(ub)); /* keeps ub + 4 sl one add */
@@ -654,7 +655,7 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   float dsc = c_dsc[0];
   uint32_t st = st_h[j];
 
-  while (alive_m != 0ull) {
+  for (;;) { /* ends in the route close that finishes the last ant */
     const uint32_t sl = s & 31u;
     if (sl == 0) {
       uint32_t r[4];
@@ -672,15 +673,22 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
      * aco_cuda_kernels.cu:271-322) */
     const float w = ((int)st <= room) ? sc : 0.0f;
     const uint64_t wm = ballot64(w > 0.0f);
-    const uint32_t e0 = (uint32_t)wm & ka0, e1 = (uint32_t)(wm >> 32) & ka1;
+    const uint64_t e = wm & ka;
     const float Pf = pair_scan(w);
     float T = __int_as_float(__builtin_amdgcn_ds_bpermute(tb, __float_as_int(Pf)));
     const uint64_t mm = ballot64(Pf > u0 * T);
-    const uint32_t m0 = (uint32_t)mm & e0, m1 = (uint32_t)(mm >> 32) & e1;
-    const uint32_t sel0 = m0 ? (uint32_t)__builtin_ctz(m0) : 31u - (uint32_t)__builtin_clz(e0 | 1u);
-    const uint32_t sel1 = m1 ? (uint32_t)__builtin_ctz(m1) : 31u - (uint32_t)__builtin_clz(e1 | 1u);
+    /* each half's first slot above its threshold; a half with none takes
+     * its last positive slot (a constructing half whose threshold rounded up
+     * to its total), or slot 0 (a half with no weight: the fallback's, or a
+     * finished ant's) */
+    const uint64_t m = mm & e;
+    int sel0 = sff1((uint32_t)m), sel1 = sff1((uint32_t)(m >> 32));
+    if (__builtin_expect((sel0 | sel1) < 0, 0)) {
+      if (sel0 < 0) sel0 = 31 - __builtin_clz((uint32_t)e | 1u);
+      if (sel1 < 0) sel1 = 31 - __builtin_clz((uint32_t)(e >> 32) | 1u);
+    }
     /* each half fetches (node | demand << 16) of its own selected lane; 0 = no move */
-    const uint32_t sela = (sel0 << 2) | ((128u + (sel1 << 2)) << 16);
+    const uint32_t sela = ((uint32_t)sel0 << 2) | ((128u + ((uint32_t)sel1 << 2)) << 16);
     const int pa = (int)__builtin_amdgcn_ubfe(sela, hs, 16u);
     uint32_t pk = (uint32_t)__builtin_amdgcn_ds_bpermute(
         pa, (int)__builtin_amdgcn_perm(st, j, 0x05040100u)); /* j | st << 16 */
@@ -692,7 +700,7 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     if (__builtin_expect((ballot64(!(T > 0.0f)) & alive_m) != 0ull, 0)) {
 #pragma unroll 1
       for (int q = 0; q < 2; ++q) {
-        if ((q ? e1 : e0) != 0u || (q ? ka1 : ka0) == 0u) continue;
+        if ((uint32_t)(e >> (32 * q)) != 0u || (uint32_t)(ka >> (32 * q)) == 0u) continue;
         const unsigned long long f0 = STATS ? clock64() : 0ull;
         const int cq = __builtin_amdgcn_readlane(cur, 32 * q);
         const int rq = __builtin_amdgcn_readlane(room, 32 * q);
@@ -783,13 +791,13 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
         }
       }
       alive_m &= ~ballot64(veh >= K);
+      if (alive_m == 0ull) break;
       if (veh >= K) { /* a finished ant's writer lane stores to its sink */
         wscale = 0;
         st_w = s_sink + lane;
         tr_w = s_sink + lane;
       }
-      ka0 = (uint32_t)alive_m & km32;
-      ka1 = (uint32_t)(alive_m >> 32) & km32;
+      ka = alive_m & km64;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -64,6 +64,15 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 __device__ __forceinline__ int first_lane(uint64_t m) { return __builtin_ctzll(m); }
+/* index of the lowest set bit of a wave-uniform word, -1 when it is 0
+ * (s_ff1_i32_b32 gives -1 for 0 itself; ffs() would add a compare and a
+ * select).  The readfirstlane costs nothing when x is already scalar and
+ * keeps the operand an SGPR when the compiler holds it in a VGPR. */
+__device__ __forceinline__ int sff1(uint32_t x) {
+  int r;
+  asm("s_ff1_i32_b32 %0, %1" : "=s"(r) : "s"(__builtin_amdgcn_readfirstlane((int)x)));
+  return r;
+}
 __device__ __forceinline__ int last_lane(uint64_t m) { return 63 - __builtin_clzll(m); }
 
 #endif /* ACO_WAVE_H */
