@@ -639,7 +639,7 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   /* lanes 0 and 32 store their constructing ant's state and tour entries,
    * the others (and those of a finished ant) to their sink */
   const bool writer = t == 0 && (h == 0 || has1);
-  int wscale = writer ? 1 : 0;
+  uint32_t wmask = writer ? ~0u : 0u; /* index mask: the sink has one entry per lane */
   uint16_t *st_w = writer ? st_h : s_sink + lane;
   uint16_t *tr_w = writer && !RING ? tr_h + 1 : s_sink + lane;
 
@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   float dsc = c_dsc[0];
   uint32_t st = st_h[j];
 
-  for (;;) { /* ends in the route close that finishes the last ant */
+  while (alive_m != 0ull) {
     const uint32_t sl = s & 31u;
     if (sl == 0) {
       uint32_t r[4];
@@ -690,8 +690,7 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     /* each half fetches (node | demand << 16) of its own selected lane; 0 = no move */
     const uint32_t sela = ((uint32_t)sel0 << 2) | ((128u + ((uint32_t)sel1 << 2)) << 16);
     const int pa = (int)__builtin_amdgcn_ubfe(sela, hs, 16u);
-    uint32_t pk = (uint32_t)__builtin_amdgcn_ds_bpermute(
-        pa, (int)__builtin_amdgcn_perm(st, j, 0x05040100u)); /* j | st << 16 */
+    uint32_t pk = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)((st << 16) | j));
     /* an ant with no feasible candidate takes the whole-wave fallback.  The
      * weights are >= 0, so a half's T (the sum of its candidates' weights) is
      * > 0 exactly when one of them is > 0: one compare over the wave finds
@@ -732,8 +731,8 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     const int md = (int)(pk >> 16);
     const uint64_t mvm = ballot64(mvw != 0u) & alive_m;
     const uint32_t st2 = st_h[j2]; /* before state[move] changes: j2 != move */
-    st_w[mvw * wscale] = (uint16_t)VISITED;
-    if constexpr (!RING) tr_w[pos * wscale] = (uint16_t)mvw;
+    st_w[mvw & wmask] = (uint16_t)VISITED;
+    if constexpr (!RING) tr_w[(uint32_t)pos & wmask] = (uint16_t)mvw;
     const int wl = (pos + 1) & 31; /* chunk slot of this decision's tour entry */
     if constexpr (RING)
       if (__builtin_amdgcn_inverse_ballot_w64(alive_m) && t == wl) ring = (int)mvw;
@@ -791,9 +790,8 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
         }
       }
       alive_m &= ~ballot64(veh >= K);
-      if (alive_m == 0ull) break;
       if (veh >= K) { /* a finished ant's writer lane stores to its sink */
-        wscale = 0;
+        wmask = 0u;
         st_w = s_sink + lane;
         tr_w = s_sink + lane;
       }
