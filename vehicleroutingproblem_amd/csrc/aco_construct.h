@@ -75,7 +75,8 @@ __device__ __forceinline__ float fb_lane_fs(float (&w)[FB_V], int b0, int n1, in
   const float fv[FB_V] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w,
                           fc.x, fc.y, fc.z, fc.w, fd.x, fd.y, fd.z, fd.w};
   const uint32_t sw[8] = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y, sb.z, sb.w};
-  const int lim = live ? room : -1;
+  int lim = live ? room : -1;
+  asm("" : "+v"(lim)); /* one compare per node against the lane's limit */
   float s = 0.0f;
 #pragma unroll
   for (int i = 0; i < FB_V; ++i) {
