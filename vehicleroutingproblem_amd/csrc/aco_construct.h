@@ -60,10 +60,12 @@ struct FbOut {
  * visited from the start, so it never carries weight).  A lane's 16 nodes are
  * contiguous, and so are its fallback weights and states. */
 
-/* Weights of the 16 nodes b..b+15 of one lane from the all-pairs weight row
- * (0 when visited, too heavy, or past the last node).  Lanes past the last
- * chunk read node 0's group and keep nothing. */
-__device__ __forceinline__ float fb_lane_fs(float (&w)[FB_V], int b0, int n1, int room,
+/* Prefixes of the 16 weights of nodes b..b+15 of one lane from the all-pairs
+ * weight row: p[i] = w[b] + ... + w[b+i], summed in slot order (a weight is 0
+ * when the node is visited, too heavy, or past the last node).  Returns the
+ * lane's sum p[15].  Lanes past the last chunk read node 0's group and keep
+ * nothing. */
+__device__ __forceinline__ float fb_lane_fs(float (&p)[FB_V], int b0, int n1, int room,
                                             const float *__restrict__ frow,
                                             const uint16_t *state) {
   const bool live = b0 < n1;
@@ -81,15 +83,27 @@ __device__ __forceinline__ float fb_lane_fs(float (&w)[FB_V], int b0, int n1, in
 #pragma unroll
   for (int i = 0; i < FB_V; ++i) {
     const int st = (int)((sw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
-    w[i] = st <= lim ? fv[i] : 0.0f;
-    s = s + w[i];
+    s = s + (st <= lim ? fv[i] : 0.0f);
+    p[i] = s;
   }
   return s;
 }
 
-/* Same weights computed from the pheromone row, the coordinates and the
- * taupow / eta tables (used when the all-pairs weight matrix is off). */
-__device__ __forceinline__ float fb_lane_tables(float (&w)[FB_V], int b0, int n1, int room,
+/* The weight of one node j from the all-pairs weight row, as fb_lane_fs
+ * computes it, and the node's state. */
+__device__ __forceinline__ float fb_node_fs(int j, int n1, int room, const float *__restrict__ frow,
+                                            const uint16_t *state, int &st) {
+  const bool live = j < n1;
+  const int jc = live ? j : 0;
+  st = (int)state[jc];
+  const float f = frow[jc];
+  return (live && st <= room) ? f : 0.0f;
+}
+
+/* Same prefixes with the weights computed from the pheromone row, the
+ * coordinates and the taupow / eta tables (used when the all-pairs weight
+ * matrix is off). */
+__device__ __forceinline__ float fb_lane_tables(float (&p)[FB_V], int b0, int n1, int room,
                                                 int2 pc, const int2 *xy, const float *taupow,
                                                 const float *eta, const uint16_t *state,
                                                 const uint8_t *__restrict__ trow, bool real) {
@@ -108,34 +122,48 @@ __device__ __forceinline__ float fb_lane_tables(float (&w)[FB_V], int b0, int n1
   for (int i = 0; i < FB_V; ++i) {
     const uint32_t d = real ? edge_code_t<true>(pc, pj[i]) : edge_code_t<false>(pc, pj[i]);
     const float v = taupow[tq[i]] * eta[d];
-    w[i] = (b0 + i < n1 && (int)st[i] <= room) ? v : 0.0f;
-    s = s + w[i];
+    s = s + ((b0 + i < n1 && (int)st[i] <= room) ? v : 0.0f);
+    p[i] = s;
   }
   return s;
+}
+
+__device__ __forceinline__ float fb_node_tables(int j, int n1, int room, int2 pc, const int2 *xy,
+                                                const float *taupow, const float *eta,
+                                                const uint16_t *state,
+                                                const uint8_t *__restrict__ trow, bool real,
+                                                int &st) {
+  const bool live = j < n1;
+  const int jc = live ? j : 0;
+  st = (int)state[jc];
+  const uint32_t d = real ? edge_code_t<true>(pc, xy[jc]) : edge_code_t<false>(pc, xy[jc]);
+  const float v = taupow[trow[jc]] * eta[d];
+  return (live && st <= room) ? v : 0.0f;
 }
 
 /* Fallback roulette over every unvisited customer that fits (the exact
  * two-pass restatement of select_global_fallback, aco_cuda_kernels.cu:336-430):
  * chunk totals, then the chunk, the lane and the slot holding the threshold.
  * FSM selects the all-pairs weight row frow; otherwise the weights are
- * computed from trow and the tables.  Returns the node (or -1), its demand and
- * the total weight. */
+ * computed from trow and the tables.  xs: 64 bytes of LDS private to the
+ * wavefront, 16-byte aligned.  Returns the node (or -1), its demand and the
+ * total weight. */
 template <bool FSM>
 __device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow, const int2 *xy,
                                               const float *taupow, const float *eta,
                                               const uint16_t *state,
                                               const uint8_t *__restrict__ trow, int n1, int cur,
-                                              int room, float u, int lane, bool real) {
+                                              int room, float u, int lane, bool real, float *xs) {
   const int nch = (n1 + FB_CHUNK - 1) / FB_CHUNK;
   const int2 pc = xy[cur];
-  float w[FB_V];
+  float p[FB_V];
   float Rv = 0.0f, Cv = 0.0f, R = 0.0f, s = 0.0f, Pl = 0.0f;
   for (int c = 0; c < nch; ++c) {
     const int b0 = c * FB_CHUNK + lane * FB_V;
     if constexpr (FSM)
-      s = fb_lane_fs(w, b0, n1, room, frow, state);
+      s = fb_lane_fs(p, b0, n1, room, frow, state);
     else
-      s = fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow, real);
+      s = fb_lane_tables(p, b0, n1, room, pc, xy, taupow, eta, state, trow, real);
     Pl = wave_scan_rows(s, lane, 4);
     const float Cc = f_readlane(Pl, 63);
     R = R + Cc;
@@ -157,30 +185,60 @@ __device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow,
   if (cs != nch - 1) {
     const int b0 = cs * FB_CHUNK + lane * FB_V;
     if constexpr (FSM)
-      s = fb_lane_fs(w, b0, n1, room, frow, state);
+      s = fb_lane_fs(p, b0, n1, room, frow, state);
     else
-      s = fb_lane_tables(w, b0, n1, room, pc, xy, taupow, eta, state, trow, real);
+      s = fb_lane_tables(p, b0, n1, room, pc, xy, taupow, eta, state, trow, real);
     Pl = wave_scan_rows(s, lane, 4);
   }
   const uint64_t lm = ballot64(s > 0.0f && Pl > thr_c);
   const int ls = lm ? first_lane(lm) : last_lane(ballot64(s > 0.0f));
   const float pprev = ls > 0 ? f_readlane(Pl, ls - 1) : 0.0f;
   const float thr_l = thr_c - pprev;
-  int is = -1, il = -1;
-  float p = 0.0f;
-#pragma unroll
-  for (int i = 0; i < FB_V; ++i) {
-    p = p + w[i];
-    if (w[i] > 0.0f) {
-      il = i;
-      if (is < 0 && p > thr_l) is = i;
-    }
+  /* The slot of lane ls: the first slot i with w[i] > 0 and p[i] > thr_l,
+   * else the last slot with w[i] > 0 (lane ls has a positive sum, so one
+   * exists).  Lane ls publishes its 16 prefixes, lane q < 16 recomputes the
+   * weight of slot q, and two ballots answer both questions at once. */
+  if (lane == ls) {
+    float4 *x4 = reinterpret_cast<float4 *>(xs);
+    x4[0] = make_float4(p[0], p[1], p[2], p[3]);
+    x4[1] = make_float4(p[4], p[5], p[6], p[7]);
+    x4[2] = make_float4(p[8], p[9], p[10], p[11]);
+    x4[3] = make_float4(p[12], p[13], p[14], p[15]);
   }
-  if (is < 0) is = il;
-  const int isel = __builtin_amdgcn_readlane(is, ls);
-  out.move = cs * FB_CHUNK + ls * FB_V + isel;
-  out.mdem = __builtin_amdgcn_readfirstlane((int)state[out.move]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int q = lane & (FB_V - 1);
+  const int node0 = cs * FB_CHUNK + ls * FB_V;
+  int stq;
+  float wq;
+  if constexpr (FSM)
+    wq = fb_node_fs(node0 + q, n1, room, frow, state, stq);
+  else
+    wq = fb_node_tables(node0 + q, n1, room, pc, xy, taupow, eta, state, trow, real, stq);
+  const float pq = xs[q];
+  const uint32_t wm = (uint32_t)ballot64(lane < FB_V && wq > 0.0f);
+  const uint32_t sm = wm & (uint32_t)ballot64(pq > thr_l);
+  const int isel = sm ? (int)__builtin_ctz(sm) : 31 - (int)__builtin_clz(wm | 1u);
+  out.move = node0 + isel;
+  out.mdem = __builtin_amdgcn_readlane(stq, isel);
   return out;
+}
+
+/* 32-bit LDS byte address of an LDS pointer; a u16 store to one. */
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char *)p;
+}
+__device__ __forceinline__ void lds_store_u16(uint32_t addr, uint16_t v) {
+  *(__attribute__((address_space(3))) uint16_t *)(uintptr_t)addr = v;
+}
+/* a * b + c on the low 24 bits of a and b (both operands < 2^24): one
+ * v_mad_u32_u24, which the compiler does not pick when it cannot prove a's
+ * range itself */
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
 }
 
 /* *(base + byte_off): a 32-bit byte offset from a wave-uniform base, so a
@@ -391,7 +449,8 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
         const unsigned long long f0 = STATS ? clock64() : 0ull;
         const FbOut fb = fallback_select<FSM>(fsm ? fsm + (size_t)cur * Sn : nullptr, c_xy, c_taupow,
                                          c_eta, s_state, tau + (size_t)cur * Sn, n1, cur, room,
-                                         u0, lane, P.dist_real != 0);
+                                         u0, lane, P.dist_real != 0,
+                                         reinterpret_cast<float *>(s_sink));
         move = fb.move;
         mdem = fb.mdem;
         T = fb.T;
@@ -632,17 +691,21 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   const uint32_t t2 = 2u * (uint32_t)t, t4 = 4u * (uint32_t)t;
 
   /* the lane's ant's state.  pos = moves + routes closed, so the customers
-   * left are lim - pos with lim = n + routes closed; close allowed iff
-   * slack = close_cap - rem_dem >= 0 with close_cap = (K - route - 1) Q
-   * (aco_cuda_kernels.cu:501-520) */
+   * left are lim - pos with lim = n + routes closed.  The depot close is
+   * allowed iff slack = close_cap - rem_dem >= 0 with close_cap =
+   * (K - route - 1) Q (aco_cuda_kernels.cu:501-520).  A move of demand d
+   * raises slack by d and lowers room by d, so slack + room is constant
+   * within a route: the test is room <= cl with cl = slack + room, which
+   * changes only when a route closes (cl -= room, then room = Q). */
   int cur = 0, room = Q, pos = 0, veh = 0, lim = n;
-  int slack = (K - 1) * Q - (int)I.total_demand;
-  /* lanes 0 and 32 store their constructing ant's state and tour entries,
-   * the others (and those of a finished ant) to their sink */
+  int cl = K * Q - (int)I.total_demand;
+  /* lanes 0 and 32 store their constructing ant's state and tour entries
+   * (byte scale 2 per index), the others (and those of a finished ant) to
+   * their own sink entry (scale 0): one multiply-add per store address */
   const bool writer = t == 0 && (h == 0 || has1);
-  uint32_t wmask = writer ? ~0u : 0u; /* index mask: the sink has one entry per lane */
-  uint16_t *st_w = writer ? st_h : s_sink + lane;
-  uint16_t *tr_w = writer && !RING ? tr_h + 1 : s_sink + lane;
+  uint32_t wsc = writer ? 2u : 0u;
+  uint32_t st_w = lds_addr(writer ? st_h : s_sink + lane);
+  uint32_t tr_w = lds_addr(writer && !RING ? tr_h + 1 : s_sink + lane);
 
   const unsigned long long t_start = STATS ? clock64() : 0ull;
   unsigned long long fb_n = 0, fb_cyc = 0, t_done = 0;
@@ -650,11 +713,14 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   uint32_t s = 0;
   float rf0 = 0.0f, rf1 = 0.0f; /* half h, lane t: draws of decision (s & ~31) + t of ant h */
 
-  /* the depot rows of both ants */
+  /* the depot rows of both ants.  States are u16 loads; the empty asm
+   * makes each an opaque 32-bit value, so the loop-carried state needs no
+   * 16-bit mask at its uses */
   uint32_t j = c_idx[t];
   float sc = c_score[t];
   float dsc = c_dsc[0];
   uint32_t st = st_h[j];
+  asm("" : "+v"(st));
 
   while (alive_m != 0ull) {
     const uint32_t sl = s & 31u;
@@ -688,10 +754,12 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
       if (sel0 < 0) sel0 = 31 - __builtin_clz((uint32_t)e | 1u);
       if (sel1 < 0) sel1 = 31 - __builtin_clz((uint32_t)(e >> 32) | 1u);
     }
-    /* each half fetches (node | demand << 16) of its own selected lane; 0 = no move */
+    /* each half fetches the node and the demand of its own selected lane;
+     * node 0 = no move */
     const uint32_t sela = ((uint32_t)sel0 << 2) | ((128u + ((uint32_t)sel1 << 2)) << 16);
     const int pa = (int)__builtin_amdgcn_ubfe(sela, hs, 16u);
-    uint32_t pk = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)((st << 16) | j));
+    uint32_t mvn = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)j);
+    int md = __builtin_amdgcn_ds_bpermute(pa, (int)st);
     /* an ant with no feasible candidate takes the whole-wave fallback.  The
      * weights are >= 0, so a half's T (the sum of its candidates' weights) is
      * > 0 exactly when one of them is > 0: one compare over the wave finds
@@ -707,9 +775,11 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
         const float uq = f_readlane(u0, 32 * q);
         const FbOut fb = fallback_select<FSM>(fsm ? fsm + (size_t)cq * Sn : nullptr, c_xy, c_taupow,
                                          c_eta, s_state0 + (size_t)q * Sn,
-                                         tau + (size_t)cq * Sn, n1, cq, rq, uq, lane, real);
+                                         tau + (size_t)cq * Sn, n1, cq, rq, uq, lane, real,
+                                         reinterpret_cast<float *>(s_sink));
         if (h == q) {
-          pk = fb.move > 0 ? ((uint32_t)fb.move | ((uint32_t)fb.mdem << 16)) : 0u;
+          mvn = fb.move > 0 ? (uint32_t)fb.move : 0u;
+          md = fb.move > 0 ? fb.mdem : 0;
           T = fb.T;
           if (STATS) {
             fb_cyc += clock64() - f0;
@@ -719,21 +789,22 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
       }
     }
     /* prefetch of both next rows (row 0 when nothing was found) */
-    const uint32_t mv = pk & 0xFFFFu;
-    const uint32_t j2 = ld_at(c_idx, __umul24(mv, k2) + t2);
-    const float sc2 = ld_at(c_score, __umul24(mv, k4) + t4);
-    const float dsc2 = ld_at(c_dsc, mv << 2);
+    uint32_t j2 = ld_at(c_idx, __umul24(mvn, k2) + t2);
+    asm("" : "+v"(j2));
+    const float sc2 = ld_at(c_score, __umul24(mvn, k4) + t4);
+    const float dsc2 = ld_at(c_dsc, mvn << 2);
     /* depot-close roulettes (aco_cuda_kernels.cu:501-520) on the half's values */
     const float thr2 = u1 * (T + dsc);
-    if (__float_as_int(dsc) > 0 && thr2 >= T && slack >= 0) pk = 0u;
+    const bool closes = (__float_as_int(dsc) > 0) & (thr2 >= T) & (room <= cl);
     /* the moves' bookkeeping: with no move lane 0 marks the depot visited
      * again and writes tour[pos + 1] = 0, which the route close writes anyway */
-    const uint32_t mvw = pk & 0xFFFFu;
-    const int md = (int)(pk >> 16);
+    const uint32_t mvw = closes ? 0u : mvn;
+    const int mdw = closes ? 0 : md;
     const uint64_t mvm = ballot64(mvw != 0u) & alive_m;
-    const uint32_t st2 = st_h[j2]; /* before state[move] changes: j2 != move */
-    st_w[mvw & wmask] = (uint16_t)VISITED;
-    if constexpr (!RING) tr_w[(uint32_t)pos & wmask] = (uint16_t)mvw;
+    uint32_t st2 = st_h[j2]; /* before state[move] changes: j2 != move */
+    asm("" : "+v"(st2));
+    lds_store_u16(mad24(mvw, wsc, st_w), (uint16_t)VISITED);
+    if constexpr (!RING) lds_store_u16(mad24((uint32_t)pos, wsc, tr_w), (uint16_t)mvw);
     const int wl = (pos + 1) & 31; /* chunk slot of this decision's tour entry */
     if constexpr (RING)
       if (__builtin_amdgcn_inverse_ballot_w64(alive_m) && t == wl) ring = (int)mvw;
@@ -744,9 +815,9 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     sc = sc2;
     dsc = dsc2;
     st = st2;
-    room -= md;
-    slack += md;
-    pos += __builtin_amdgcn_inverse_ballot_w64(mvm) ? 1 : 0;
+    room -= mdw;
+    /* pos += 1 in the lanes of mvm: one add with the mask as its carry-in */
+    asm("v_addc_co_u32_e64 %0, vcc, 0, %0, %1" : "+v"(pos) : "s"(mvm) : "vcc");
     cur = (int)mvw;
     if constexpr (RING) { /* a move that completes a chunk stores it */
       const uint64_t fl = mvm & ballot64(wl == 31);
@@ -783,8 +854,8 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
           if (STATS) t_done = clock64();
         } else {
           cur = 0;
+          cl -= room;
           room = Q;
-          slack -= Q;
           j = c_idx[t];
           sc = c_score[t];
           dsc = c_dsc[0];
@@ -792,15 +863,16 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
       }
       alive_m &= ~ballot64(veh >= K);
       if (veh >= K) { /* a finished ant's writer lane stores to its sink */
-        wmask = 0u;
-        st_w = s_sink + lane;
-        tr_w = s_sink + lane;
+        wsc = 0u;
+        st_w = lds_addr(s_sink + lane);
+        tr_w = st_w;
       }
       ka = alive_m & km64;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       st = st_h[j];
+      asm("" : "+v"(st));
     }
   }
   const int rem_cnt = lim - pos;
