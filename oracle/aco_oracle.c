@@ -518,7 +518,11 @@ int oracle_construct_ant(const OracleColony *o, int ant, int epoch, int32_t *tou
         }
       }
       oracle_scan_rows(w, k, P);
-      float T = P[k - 1];
+      /* the total: the prefix at the last lane of the last 16-lane row the
+       * list spans, the weights past k being 0 (for k a multiple of 16 the
+       * prefix of the last candidate); the rows' in-row scans fix its
+       * grouping, so it can differ from P[k - 1] in the last bit */
+      float T = P[16 * ((k + 15) / 16) - 1];
       int move = -1, mdem = 0;
       if (T > 0.0f) {
         float thr = u01(r[0]) * T;

@@ -431,15 +431,18 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
       ++s;
       const float u0 = f_readlane(rf0, sl);
 
-      /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322) */
-      const float w = ((int)st <= room) ? sc : 0.0f;
-      const uint64_t emask = ballot64(w > 0.0f) & kmask;
+      /* candidate roulette (select_from_candidates, aco_cuda_kernels.cu:271-322).
+       * Lanes past the list weigh 0, so the total T is the prefix at the
+       * last lane of the last row, which equals that of the instance's own
+       * last row when the launch spans more rows than its list */
+      const float w = (((int)st <= room) & __builtin_amdgcn_inverse_ballot_w64(kmask)) ? sc : 0.0f;
+      const uint64_t emask = ballot64(w > 0.0f);
       const float Pf = cand_scan<ROWS>(w, lane);
       int move, mdem;
       float T;
       if (__builtin_expect(emask != 0, 1)) {
         /* T > 0 exactly when some weight is > 0 (sums of non-negatives) */
-        T = f_readlane(Pf, k - 1);
+        T = f_readlane(Pf, 16 * ROWS - 1);
         const float thr = u0 * T;
         const uint64_t m = ballot64(Pf > thr) & emask;
         const int sel = m ? first_lane(m) : last_lane(emask);
@@ -549,6 +552,15 @@ __device__ __forceinline__ float pair_scan(float v) {
   asm("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf"
       : "+v"(v));
   return v;
+}
+
+/* Each half's total in all of its 32 lanes: lane 15 of every row to the
+ * whole row (row_newbcast:15), then rows 1 and 3 over rows 0 and 2
+ * (v_permlane16_swap: its second result holds the odd rows of the first
+ * operand in both rows of each pair).  Two VALU operations, no LDS. */
+__device__ __forceinline__ float pair_total(float v) {
+  const int b = __builtin_amdgcn_mov_dpp(__float_as_int(v), 0x15F, 0xF, 0xF, false);
+  return __int_as_float(__builtin_amdgcn_permlane16_swap(b, b, false, false)[1]);
 }
 
 /* Two ants per wavefront, for candidate lists of at most 32: lanes 32h+t
@@ -682,10 +694,9 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   /* lanes of the ants still constructing, and their candidate slots per half */
   uint64_t alive_m = has1 ? ~0ull : 0xFFFFFFFFull;
   uint64_t ka = alive_m & km64;
-  /* byte addresses for ds_bpermute: lane 32h, and this half's last candidate */
+  /* byte address for ds_bpermute: lane 32h */
   int ub = (lane & 32) << 2;
   asm("" : "+v"(ub)); /* keeps ub + 4 sl one add */
-  const int tb = ((lane & 32) + k - 1) << 2;
   const uint32_t hs = (uint32_t)h << 4;
   const uint32_t k2 = 2u * (uint32_t)k, k4 = 4u * (uint32_t)k;
   const uint32_t t2 = 2u * (uint32_t)t, t4 = 4u * (uint32_t)t;
@@ -738,11 +749,13 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
 
     /* candidate roulettes of both ants (select_from_candidates,
      * aco_cuda_kernels.cu:271-322) */
-    const float w = ((int)st <= room) ? sc : 0.0f;
+    /* slots past k weigh 0, so lane 31 of a half holds its total T: the
+     * prefix at the last lane of the half's second row */
+    const float w = (((int)st <= room) & __builtin_amdgcn_inverse_ballot_w64(km64)) ? sc : 0.0f;
     const uint64_t wm = ballot64(w > 0.0f);
     const uint64_t e = wm & ka;
     const float Pf = pair_scan(w);
-    float T = __int_as_float(__builtin_amdgcn_ds_bpermute(tb, __float_as_int(Pf)));
+    float T = pair_total(Pf);
     const uint64_t mm = ballot64(Pf > u0 * T);
     /* each half's first slot above its threshold; a half with none takes
      * its last positive slot (a constructing half whose threshold rounded up
