@@ -225,6 +225,7 @@ def batched_leg(epochs: int, fallback_matrix: int = -1):
             "cand_k_by_size_class": {str(c): k for c, k in sorted(k_table.items())} or "16",
             "buckets": [{"n1": [b["n1_min"], b["n1_max"]], "instances": b["instances"],
                          "ants_per_group": b["ants_per_group"], "stage_lds": b["stage_lds"],
+                         "ants_per_wave": b["ants_per_wave"], "state_bytes": b["state_bytes"],
                          "construct_ms": round(b["tuned_ms"], 3)} for b in shapes],
             "prepare_s": round(t_prep, 3), "autotune_s": round(t_tune, 3)}
 

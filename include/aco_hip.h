@@ -184,11 +184,14 @@ typedef struct {
   int lds_bytes;         /* LDS per workgroup */
   float tuned_ms;        /* construct time of the chosen shape (autotuner), 0 = not tuned */
   int ants_per_wave;     /* 1, or 2: two ants per wavefront (candidate lists <= 32) */
+  int state_bytes;       /* bytes per node of an ant's LDS state: 2, or 1 for pairs whose
+                          * instances' demands are all <= 254 */
 } AcoHipBucketInfo;
 int aco_hip_num_buckets(const AcoHipContext *ctx);
 int aco_hip_get_bucket(const AcoHipContext *ctx, int bucket, AcoHipBucketInfo *info);
 /* Force a bucket's shape (stage_lds -1 = automatic); the effective shape may
- * have fewer ants per workgroup when LDS does not fit the request.  Ants per
+ * have fewer ants per workgroup when LDS does not fit the request (at most 8
+ * wavefronts per workgroup, 16 for pairs with byte states).  Ants per
  * wavefront stay automatic. */
 int aco_hip_set_bucket_shape(AcoHipContext *ctx, int bucket, int ants_per_group, int stage_lds);
 /* The same with the ants per wavefront: 0 automatic, 1, or 2 (two ants per
@@ -197,7 +200,7 @@ int aco_hip_set_bucket_shape(AcoHipContext *ctx, int bucket, int ants_per_group,
 int aco_hip_set_bucket_layout(AcoHipContext *ctx, int bucket, int ants_per_group, int stage_lds,
                               int ants_per_wave);
 /* Time `launches` construct launches of every bucket for each wavefronts-
- * per-workgroup in {1, 2, 4, 8}, each staging mode that fits and one or two
+ * per-workgroup in {1, 2, 4, 8, 16}, each staging mode that fits and one or two
  * ants per wavefront, apply the fastest per bucket and report them (info:
  * max_buckets entries).  Colonies must not be halted; pheromone and colony
  * state are left unchanged. */

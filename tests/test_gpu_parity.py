@@ -216,6 +216,60 @@ def test_pair_layout_ragged_ant_counts(gpu, ants, apg, ring, monkeypatch):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("name", PAIR_CASES)
+@pytest.mark.parametrize("stage,fm,ring", [(0, 1, 0), (0, 1, 1), (1, 1, 1), (0, 0, 1)])
+def test_pair_byte_states_match_oracle(gpu, name, stage, fm, ring, monkeypatch):
+    """Two ants per wavefront with byte states (every demand of these
+    instances is at most 254; the room a state is compared with is capped at
+    254, below the visited mark 255): four epochs give the oracle's tours,
+    costs, colony state and pheromone, with rows staged or global, tours in
+    LDS or in registers, with and without the all-pairs fallback weights.
+    n300_clustered and n1030_two_chunks have capacities above 254, so the
+    cap is exercised."""
+    monkeypatch.setenv("ACO_HIP_STATE8", "1")
+    monkeypatch.setenv("ACO_HIP_TOUR_RING", str(ring))
+    inst, s, o = make_pair(name, fallback_matrix=fm)
+    try:
+        s.set_bucket_shape(0, 4, stage, ants_per_wave=2)
+    except AcoError:
+        pytest.skip("staged rows do not fit the LDS with two ants per wavefront")
+    b = s.buckets()[0]
+    assert b["ants_per_wave"] == 2 and b["state_bytes"] == 1
+    assert s.stage_lds == stage
+    s.prepare()
+    for e in range(4):
+        s.epochs(1)
+        o.epochs(1)
+        _compare_tours(s, o)
+        gs, os_ = s.colony_state(), o.state()
+        for key in ("epoch", "best_ant", "best_len", "feasible_ants", "global_best_cost"):
+            assert gs[key] == os_[key], (e, key, gs[key], os_[key])
+    tg, cg = s.tau()
+    to, co = o.tau()
+    assert np.array_equal(tg, to) and np.array_equal(cg, co)
+    s.close()
+
+
+def test_pair_byte_states_refused_for_heavy_demands(gpu, monkeypatch):
+    """A demand above 254 keeps u16 states even when byte states are asked
+    for, and the tours still equal the oracle's."""
+    monkeypatch.setenv("ACO_HIP_STATE8", "1")
+    inst = Instance.uniform(120, 6, dmin=200, dmax=400, seed=17)
+    assert int(np.max(inst.demand)) > 254
+    p = Params(cand_k=16, ants=32, rho=0.2)
+    with HipSolver(inst, p, seeds=[5]) as s:
+        s.set_bucket_shape(0, 2, 0, ants_per_wave=2)
+        b = s.buckets()[0]
+        assert b["ants_per_wave"] == 2 and b["state_bytes"] == 2
+        o = O.Oracle(inst, cand_k=16, ants=32, rho=0.2, seed=5)
+        s.prepare()
+        s.epochs(3)
+        o.epochs(3)
+        _compare_tours(s, o)
+        assert s.colony_state()["global_best_cost"] == o.state()["global_best_cost"]
+        o.close()
+
+
 @pytest.mark.parametrize("name,fm", [("c1_n100", 1), ("n300_clustered", 1), ("smoke_unit", 1),
                                      ("n1030_two_chunks", 1), ("n1030_two_chunks", 0),
                                      ("n2100_k6_fallbacks", 1), ("short_routes", 1)])

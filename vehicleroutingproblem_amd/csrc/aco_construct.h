@@ -21,6 +21,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "aco_inline.h"
 #include "aco_kernels.h"
 #include "aco_wave.h"
@@ -65,24 +67,35 @@ struct FbOut {
  * when the node is visited, too heavy, or past the last node).  Returns the
  * lane's sum p[15].  Lanes past the last chunk read node 0's group and keep
  * nothing. */
+template <class SE>
 __device__ __forceinline__ float fb_lane_fs(float (&p)[FB_V], int b0, int n1, int room,
-                                            const float *__restrict__ frow,
-                                            const uint16_t *state) {
+                                            const float *__restrict__ frow, const SE *state) {
   const bool live = b0 < n1;
   const int b = live ? b0 : 0;
   const float4 *f4 = reinterpret_cast<const float4 *>(frow + b);
   const uint4 *s4 = reinterpret_cast<const uint4 *>(state + b);
   const float4 fa = f4[0], fb = f4[1], fc = f4[2], fd = f4[3];
-  const uint4 sa = s4[0], sb = s4[1];
+  /* the 16 states: two 16-byte loads of u16, or one of bytes */
+  constexpr int SB = 8 * (int)sizeof(SE);    /* bits per state */
+  constexpr int SPW = 32 / SB;               /* states per 32-bit word */
+  constexpr uint32_t SM = (1u << SB) - 1u;
+  uint32_t sw[8];
+  {
+    const uint4 sa = s4[0];
+    sw[0] = sa.x; sw[1] = sa.y; sw[2] = sa.z; sw[3] = sa.w;
+    if constexpr (sizeof(SE) == 2) {
+      const uint4 sb = s4[1];
+      sw[4] = sb.x; sw[5] = sb.y; sw[6] = sb.z; sw[7] = sb.w;
+    }
+  }
   const float fv[FB_V] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w,
                           fc.x, fc.y, fc.z, fc.w, fd.x, fd.y, fd.z, fd.w};
-  const uint32_t sw[8] = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y, sb.z, sb.w};
   int lim = live ? room : -1;
   asm("" : "+v"(lim)); /* one compare per node against the lane's limit */
   float s = 0.0f;
 #pragma unroll
   for (int i = 0; i < FB_V; ++i) {
-    const int st = (int)((sw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
+    const int st = (int)((sw[i / SPW] >> (SB * (i % SPW))) & SM);
     s = s + (st <= lim ? fv[i] : 0.0f);
     p[i] = s;
   }
@@ -91,8 +104,9 @@ __device__ __forceinline__ float fb_lane_fs(float (&p)[FB_V], int b0, int n1, in
 
 /* The weight of one node j from the all-pairs weight row, as fb_lane_fs
  * computes it, and the node's state. */
+template <class SE>
 __device__ __forceinline__ float fb_node_fs(int j, int n1, int room, const float *__restrict__ frow,
-                                            const uint16_t *state, int &st) {
+                                            const SE *state, int &st) {
   const bool live = j < n1;
   const int jc = live ? j : 0;
   st = (int)state[jc];
@@ -103,9 +117,10 @@ __device__ __forceinline__ float fb_node_fs(int j, int n1, int room, const float
 /* Same prefixes with the weights computed from the pheromone row, the
  * coordinates and the taupow / eta tables (used when the all-pairs weight
  * matrix is off). */
+template <class SE>
 __device__ __forceinline__ float fb_lane_tables(float (&p)[FB_V], int b0, int n1, int room,
                                                 int2 pc, const int2 *xy, const float *taupow,
-                                                const float *eta, const uint16_t *state,
+                                                const float *eta, const SE *state,
                                                 const uint8_t *__restrict__ trow, bool real) {
   uint32_t st[FB_V], tq[FB_V];
   int2 pj[FB_V];
@@ -128,9 +143,10 @@ __device__ __forceinline__ float fb_lane_tables(float (&p)[FB_V], int b0, int n1
   return s;
 }
 
+template <class SE>
 __device__ __forceinline__ float fb_node_tables(int j, int n1, int room, int2 pc, const int2 *xy,
                                                 const float *taupow, const float *eta,
-                                                const uint16_t *state,
+                                                const SE *state,
                                                 const uint8_t *__restrict__ trow, bool real,
                                                 int &st) {
   const bool live = j < n1;
@@ -145,13 +161,14 @@ __device__ __forceinline__ float fb_node_tables(int j, int n1, int room, int2 pc
  * two-pass restatement of select_global_fallback, aco_cuda_kernels.cu:336-430):
  * chunk totals, then the chunk, the lane and the slot holding the threshold.
  * FSM selects the all-pairs weight row frow; otherwise the weights are
- * computed from trow and the tables.  xs: 64 bytes of LDS private to the
+ * computed from trow and the tables.  States are u16 or bytes (SE), the
+ * visited mark above any room passed.  xs: 64 bytes of LDS private to the
  * wavefront, 16-byte aligned.  Returns the node (or -1), its demand and the
  * total weight. */
-template <bool FSM>
+template <bool FSM, class SE>
 __device__ __forceinline__ FbOut fallback_select(const float *__restrict__ frow, const int2 *xy,
                                               const float *taupow, const float *eta,
-                                              const uint16_t *state,
+                                              const SE *state,
                                               const uint8_t *__restrict__ trow, int n1, int cur,
                                               int room, float u, int lane, bool real, float *xs) {
   const int nch = (n1 + FB_CHUNK - 1) / FB_CHUNK;
@@ -231,6 +248,9 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p) {
 }
 __device__ __forceinline__ void lds_store_u16(uint32_t addr, uint16_t v) {
   *(__attribute__((address_space(3))) uint16_t *)(uintptr_t)addr = v;
+}
+__device__ __forceinline__ void lds_store_u8(uint32_t addr, uint8_t v) {
+  *(__attribute__((address_space(3))) uint8_t *)(uintptr_t)addr = v;
 }
 /* a * b + c on the low 24 bits of a and b (both operands < 2^24): one
  * v_mad_u32_u24, which the compiler does not pick when it cannot prove a's
@@ -450,7 +470,7 @@ __global__ void __launch_bounds__(512) k_construct(DevCtx P, int groups_per_colo
         mdem = __builtin_amdgcn_readlane((int)st, sel);
       } else {
         const unsigned long long f0 = STATS ? clock64() : 0ull;
-        const FbOut fb = fallback_select<FSM>(fsm ? fsm + (size_t)cur * Sn : nullptr, c_xy, c_taupow,
+        const FbOut fb = fallback_select<FSM, uint16_t>(fsm ? fsm + (size_t)cur * Sn : nullptr, c_xy, c_taupow,
                                          c_eta, s_state, tau + (size_t)cur * Sn, n1, cur, room,
                                          u0, lane, P.dist_real != 0,
                                          reinterpret_cast<float *>(s_sink));
@@ -574,10 +594,18 @@ __device__ __forceinline__ float pair_total(float v) {
  * rule, fallback and close rule), so the tours are bit-identical: both ants
  * of a wavefront decide in lockstep, one decision each per iteration, and an
  * ant whose tour is complete idles until its partner's is too.  Ant
- * 2(gW + w) + h of the colony is half h of wavefront w of workgroup g. */
-template <bool STAGE, bool STATS, bool XCD, bool FSM, bool RING>
-__global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per_colony, int W,
-                                                        const int *cols) {
+ * 2(gW + w) + h of the colony is half h of wavefront w of workgroup g.
+ * B8: byte states (demand, 0xFF once visited) for instances whose demands
+ * are all at most 254, half the LDS of u16 states; the room a state is
+ * compared with is then capped at 254, which keeps every test exact.  With
+ * B8 the staged layout leaves the coordinates in global memory (they are
+ * read only for the tour cost), so a workgroup of up to 16 wavefronts can
+ * share one staged copy of an n=1000 colony's candidate rows. */
+template <bool STAGE, bool STATS, bool XCD, bool FSM, bool RING, bool B8>
+__global__ void __launch_bounds__(B8 ? 1024 : 512) k_construct_pair(DevCtx P, int groups_per_colony,
+                                                                   int W, const int *cols) {
+  typedef typename std::conditional<B8, uint8_t, uint16_t>::type SE;
+  constexpr uint32_t VIS = B8 ? 0xFFu : (uint32_t)VISITED;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const unsigned long long t_kernel = STATS ? clock64() : 0ull;
   unsigned lb = blockIdx.x;
@@ -609,7 +637,7 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   const uint16_t *c_idx;
   const float *c_score;
   const float *c_dsc;
-  const int2 *c_xy;
+  const int2 *c_xy = P.xy + I.node_off;
   if (STAGE) {
     uint16_t *s_idx = reinterpret_cast<uint16_t *>(smem + off);
     off += round16(ent * 2);
@@ -617,21 +645,22 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     off += round16(ent * 4);
     float *s_dsc = reinterpret_cast<float *>(smem + off);
     off += round16((size_t)n1 * 4);
-    int2 *s_xy = reinterpret_cast<int2 *>(smem + off);
-    off += round16((size_t)n1 * 8);
     wg_copy16(s_idx, P.cidx + I.cand_off, round16(ent * 2));
     wg_copy16(s_score, P.score + D.score_off, round16(ent * 4));
     wg_copy16(s_dsc, P.dscore + I.node_off, round16((size_t)n1 * 4));
-    wg_copy16(s_xy, P.xy + I.node_off, round16((size_t)n1 * 8));
+    if constexpr (!B8) {
+      int2 *s_xy = reinterpret_cast<int2 *>(smem + off);
+      off += round16((size_t)n1 * 8);
+      wg_copy16(s_xy, P.xy + I.node_off, round16((size_t)n1 * 8));
+      c_xy = s_xy;
+    }
     c_idx = s_idx;
     c_score = s_score;
     c_dsc = s_dsc;
-    c_xy = s_xy;
   } else {
     c_idx = P.cidx + I.cand_off;
     c_score = P.score + D.score_off;
     c_dsc = P.dscore + I.node_off;
-    c_xy = P.xy + I.node_off;
   }
   const float *c_taupow = P.taupow;
   const float *c_eta = P.eta;
@@ -647,8 +676,8 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
       c_eta = s_eta;
     }
   }
-  uint16_t *const s_state0 = reinterpret_cast<uint16_t *>(smem + off) + (size_t)wave * 2 * Sn;
-  off += (size_t)W * 2 * Sn * 2;
+  SE *const s_state0 = reinterpret_cast<SE *>(smem + off) + (size_t)wave * 2 * Sn;
+  off += (size_t)W * 2 * Sn * sizeof(SE);
   /* without RING, two LDS tour buffers per wave */
   const size_t tour_elems = round16((size_t)D.stride * 2) / 2;
   uint16_t *const s_tour0 =
@@ -672,13 +701,13 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
    * 0, the start depot, is 0), and a chunk is stored by its 32 lanes once
    * complete, so no line is written back half-filled and LDS holds only the
    * two state arrays. */
-  uint16_t *const st_h = s_state0 + (size_t)h * Sn;
+  SE *const st_h = s_state0 + (size_t)h * Sn;
   int32_t *const tour_h = P.tours + D.tour_off + (size_t)ant_h * D.stride;
   uint16_t *const tr_h = RING ? nullptr : s_tour0 + (size_t)h * tour_elems;
   int ring = 0;
   const uint16_t *g_dem = P.dem + I.node_off;
   for (int j = t; j < Sn; j += 32)
-    st_h[j] = (j == 0 || j >= n1) ? (uint16_t)VISITED : g_dem[j];
+    st_h[j] = (j == 0 || j >= n1) ? (SE)VIS : (SE)g_dem[j];
   if (!RING && t == 0) tr_h[0] = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -711,11 +740,13 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
   int cur = 0, room = Q, pos = 0, veh = 0, lim = n;
   int cl = K * Q - (int)I.total_demand;
   /* lanes 0 and 32 store their constructing ant's state and tour entries
-   * (byte scale 2 per index), the others (and those of a finished ant) to
-   * their own sink entry (scale 0): one multiply-add per store address */
+   * (byte scale sizeof(SE) or 2 per index), the others (and those of a
+   * finished ant) to their own sink entry (scale 0): one multiply-add per
+   * store address */
   const bool writer = t == 0 && (h == 0 || has1);
-  uint32_t wsc = writer ? 2u : 0u;
-  uint32_t st_w = lds_addr(writer ? st_h : s_sink + lane);
+  uint32_t wsc = writer ? (uint32_t)sizeof(SE) : 0u;
+  uint32_t wsc2 = writer ? 2u : 0u;
+  uint32_t st_w = writer ? lds_addr(st_h) : lds_addr(s_sink + lane);
   uint32_t tr_w = lds_addr(writer && !RING ? tr_h + 1 : s_sink + lane);
 
   const unsigned long long t_start = STATS ? clock64() : 0ull;
@@ -751,7 +782,8 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
      * aco_cuda_kernels.cu:271-322) */
     /* slots past k weigh 0, so lane 31 of a half holds its total T: the
      * prefix at the last lane of the half's second row */
-    const float w = (((int)st <= room) & __builtin_amdgcn_inverse_ballot_w64(km64)) ? sc : 0.0f;
+    const int rl = B8 ? min(room, 254) : room;
+    const float w = (((int)st <= rl) & __builtin_amdgcn_inverse_ballot_w64(km64)) ? sc : 0.0f;
     const uint64_t wm = ballot64(w > 0.0f);
     const uint64_t e = wm & ka;
     const float Pf = pair_scan(w);
@@ -784,9 +816,9 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
         if ((uint32_t)(e >> (32 * q)) != 0u || (uint32_t)(ka >> (32 * q)) == 0u) continue;
         const unsigned long long f0 = STATS ? clock64() : 0ull;
         const int cq = __builtin_amdgcn_readlane(cur, 32 * q);
-        const int rq = __builtin_amdgcn_readlane(room, 32 * q);
+        const int rq = __builtin_amdgcn_readlane(rl, 32 * q);
         const float uq = f_readlane(u0, 32 * q);
-        const FbOut fb = fallback_select<FSM>(fsm ? fsm + (size_t)cq * Sn : nullptr, c_xy, c_taupow,
+        const FbOut fb = fallback_select<FSM, SE>(fsm ? fsm + (size_t)cq * Sn : nullptr, c_xy, c_taupow,
                                          c_eta, s_state0 + (size_t)q * Sn,
                                          tau + (size_t)cq * Sn, n1, cq, rq, uq, lane, real,
                                          reinterpret_cast<float *>(s_sink));
@@ -816,8 +848,11 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
     const uint64_t mvm = ballot64(mvw != 0u) & alive_m;
     uint32_t st2 = st_h[j2]; /* before state[move] changes: j2 != move */
     asm("" : "+v"(st2));
-    lds_store_u16(mad24(mvw, wsc, st_w), (uint16_t)VISITED);
-    if constexpr (!RING) lds_store_u16(mad24((uint32_t)pos, wsc, tr_w), (uint16_t)mvw);
+    if constexpr (B8)
+      lds_store_u8(mad24(mvw, wsc, st_w), (uint8_t)VIS);
+    else
+      lds_store_u16(mad24(mvw, wsc, st_w), (uint16_t)VIS);
+    if constexpr (!RING) lds_store_u16(mad24((uint32_t)pos, wsc2, tr_w), (uint16_t)mvw);
     const int wl = (pos + 1) & 31; /* chunk slot of this decision's tour entry */
     if constexpr (RING)
       if (__builtin_amdgcn_inverse_ballot_w64(alive_m) && t == wl) ring = (int)mvw;
@@ -877,6 +912,7 @@ __global__ void __launch_bounds__(512) k_construct_pair(DevCtx P, int groups_per
       alive_m &= ~ballot64(veh >= K);
       if (veh >= K) { /* a finished ant's writer lane stores to its sink */
         wsc = 0u;
+        wsc2 = 0u;
         st_w = lds_addr(s_sink + lane);
         tr_w = st_w;
       }

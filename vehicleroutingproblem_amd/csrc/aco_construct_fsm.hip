@@ -17,13 +17,15 @@ ACO_CONSTRUCT_VARIANTS(3)
 ACO_CONSTRUCT_VARIANTS(4)
 #undef ACO_CONSTRUCT_VARIANTS
 
-#define ACO_PAIR_VARIANTS(RG)                                                              \
-  template __global__ void k_construct_pair<true, false, false, true, RG>(DevCtx, int, int, const int *);  \
-  template __global__ void k_construct_pair<false, false, false, true, RG>(DevCtx, int, int, const int *); \
-  template __global__ void k_construct_pair<true, true, false, true, RG>(DevCtx, int, int, const int *);   \
-  template __global__ void k_construct_pair<false, true, false, true, RG>(DevCtx, int, int, const int *);  \
-  template __global__ void k_construct_pair<true, false, true, true, RG>(DevCtx, int, int, const int *);   \
-  template __global__ void k_construct_pair<false, false, true, true, RG>(DevCtx, int, int, const int *);
-ACO_PAIR_VARIANTS(false)
-ACO_PAIR_VARIANTS(true)
+#define ACO_PAIR_VARIANTS(RG, B8)                                                          \
+  template __global__ void k_construct_pair<true, false, false, true, RG, B8>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct_pair<false, false, false, true, RG, B8>(DevCtx, int, int, const int *); \
+  template __global__ void k_construct_pair<true, true, false, true, RG, B8>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct_pair<false, true, false, true, RG, B8>(DevCtx, int, int, const int *);  \
+  template __global__ void k_construct_pair<true, false, true, true, RG, B8>(DevCtx, int, int, const int *);   \
+  template __global__ void k_construct_pair<false, false, true, true, RG, B8>(DevCtx, int, int, const int *);
+ACO_PAIR_VARIANTS(false, false)
+ACO_PAIR_VARIANTS(true, false)
+ACO_PAIR_VARIANTS(false, true)
+ACO_PAIR_VARIANTS(true, true)
 #undef ACO_PAIR_VARIANTS

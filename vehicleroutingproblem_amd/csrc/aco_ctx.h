@@ -30,6 +30,7 @@ struct LaunchBucket {
   int W = 4, stage = 1, eta_lds = 1, rows = 1, groups = 1;
   int pair = 0;              /* two ants per wavefront (k_construct_pair) */
   int ring = 0;              /* pairs: tours in registers, stored in 128-byte chunks */
+  int b8 = 0;                /* pairs: byte states (every demand of the bucket <= 254) */
   size_t lds = 0;
   float tuned_ms = 0.0f;     /* construct time measured by the autotuner, 0 = not tuned */
   size_t off_at = 0;         /* its item offsets at d_boff[off_at], slot offsets after them */
@@ -48,6 +49,8 @@ struct AcoHipContext {
   int rows = 1;        /* 16-lane rows of the widest candidate list (k_construct ROWS) */
   int pair_mode = -1;  /* ACO_HIP_PAIR: -1 automatic, 0 never, 1 whenever k <= 32 */
   int ring_mode = -1;  /* ACO_HIP_TOUR_RING: pairs' tours -1 automatic, 0 in LDS, 1 in registers */
+  int b8_mode = -1;    /* ACO_HIP_STATE8: pairs' byte states -1 / 1 when demands allow, 0 never */
+  int xcd_mode = -1;   /* ACO_HIP_XCD: XCD-aware workgroup order -1 with several colonies, 0, 1 */
   int fs_on = 0;       /* all-pairs fallback weight matrix allocated */
   size_t sd_lds = 0;   /* dynamic LDS of k_select_deposit, 0 = fused k_evap_update path */
   int sd_eta = 0;      /* k_select_deposit stages the eta table in LDS */
@@ -56,6 +59,7 @@ struct AcoHipContext {
   std::vector<InstDesc> inst;
   std::vector<ColonyDesc> col;
   std::vector<int> inst_n1;
+  std::vector<int> inst_dmax; /* largest customer demand per instance */
   AcoTauTables tables{};
   std::vector<float> eta_lut;
   int eta_len = 0;

@@ -94,21 +94,21 @@ static int fs_stride_of(int n1) { return (n1 + 15) & ~15; }
 
 /* LDS bytes of one construct workgroup for an instance of n1 nodes (layout
  * of k_construct and k_construct_pair): staged candidate rows (+64 padding
- * entries), depot scores and coordinates; the computed-fallback tables
- * (taupow, and eta unless eta_len == 0) when the weight matrix is off; per
- * wave a state array of S u16 and a tour buffer of `stride` u16, or with
- * `pair` two state arrays and (unless `ring`: the pair then stores its tours
- * in 128-byte chunks from registers) two tour buffers, and a 128-byte store
- * sink. */
+ * entries), depot scores and (unless b8) coordinates; the computed-fallback
+ * tables (taupow, and eta unless eta_len == 0) when the weight matrix is
+ * off; per wave a state array of S u16 and a tour buffer of `stride` u16,
+ * or with `pair` two state arrays (of S bytes each with b8) and (unless
+ * `ring`: the pair then stores its tours in 128-byte chunks from registers)
+ * two tour buffers, and a 128-byte store sink. */
 static size_t lds_for(int n1, int k, int eta_len, int W, int stage, int fs_on, int stride,
-                      int pair, int ring = 0) {
-  const size_t state = (size_t)fs_stride_of(n1) * 2;
+                      int pair, int ring = 0, int b8 = 0) {
+  const size_t state = (size_t)fs_stride_of(n1) * (b8 ? 1 : 2);
   const size_t tour = r16((size_t)stride * 2);
   size_t b = (size_t)W * ((pair ? 2 * state + (ring ? 0 : 2 * tour) : state + tour) + 128);
   if (!fs_on) b += 256 * 4 + r16((size_t)eta_len * 4);
   if (stage) {
     const size_t ent = (size_t)n1 * k + 64;
-    b += r16(ent * 2) + r16(ent * 4) + r16((size_t)n1 * 4) + r16((size_t)n1 * 8);
+    b += r16(ent * 2) + r16(ent * 4) + r16((size_t)n1 * 4) + (b8 ? 0 : r16((size_t)n1 * 8));
   }
   return b;
 }
@@ -222,28 +222,21 @@ static int env_int(const char *name, int dflt) {
   return (s && *s) ? atoi(s) : dflt;
 }
 
-/* Ants per workgroup, LDS staging and ants per wavefront of a bucket.
+/* Ants per workgroup and LDS staging of a bucket for the given ants per
+ * wavefront (pair) and state width (b8: byte states).
  * Staging preference: the requested wavefronts per workgroup (W_req,
- * clamped to 1..8 and the ant count), staged rows with the eta table, staged
+ * clamped to 1..8, 1..16 with byte states, and the ant count), staged rows
+ * with the eta table, staged
  * rows without it, then (unless staging was forced) global rows, then fewer
- * wavefronts per workgroup.  want_pair: 1 two ants per wavefront when the
- * candidate lists allow it (k <= 32), 0 one, -1 automatic: two when the
- * launch has more ants than the device has SIMDs (the wavefronts then share
- * SIMDs and the launch is bound by the instructions issued per ant, which a
- * pair halves); one otherwise (each ant's chain of decisions is the bound). */
-static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_stage,
-                        int want_pair = -1) {
+ * wavefronts per workgroup. */
+static int shape_for(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_stage, int pair,
+                     int b8) {
   int kmax = 1;
   for (int i : lb.insts) kmax = std::max(kmax, c->inst[i].k);
-  int pair = 0;
-  if (kmax <= 32) {
-    if (want_pair < 0) want_pair = c->pair_mode;
-    if (want_pair > 0) pair = 1;
-    else if (want_pair < 0)
-      pair = (long long)lb.cols.size() * c->prm.ants > 4LL * c->num_cu ? 1 : 0;
-  }
   const int apw = pair ? 2 : 1;
-  int W = std::max(1, std::min(8, std::min(W_req, (c->prm.ants + apw - 1) / apw)));
+  const int wmax = b8 ? 16 : 8;
+  const int wcap = std::max(1, std::min(wmax, (c->prm.ants + apw - 1) / apw));
+  int W = std::max(1, std::min(wcap, W_req));
   int stage = want_stage == 0 ? 0 : 1;
   int eta_in_lds = 1;
   int ring = 0;
@@ -256,7 +249,7 @@ static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_
       size_t b = 0;
       for (int i : lb.insts)
         b = std::max(b, lds_for(c->inst_n1[i], c->inst[i].k, eta_in_lds ? c->eta_len : 0, W,
-                                stage, c->fs_on, c->max_stride, pair, rg));
+                                stage, c->fs_on, c->max_stride, pair, rg, b8));
       return b;
     };
     ring = 0;
@@ -310,8 +303,55 @@ static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_
   lb.rows = (kmax + 15) / 16;
   lb.pair = pair;
   lb.ring = ring;
+  lb.b8 = b8;
   lb.groups = (c->prm.ants + W * apw - 1) / (W * apw);
   lb.lds = need;
+  return ACO_HIP_OK;
+}
+
+/* The bucket's shape.  want_pair: 1 two ants per wavefront when the
+ * candidate lists allow it (k <= 32), 0 one, -1 automatic: two when the
+ * launch has more ants than the device has SIMDs (the wavefronts then share
+ * SIMDs and the launch is bound by the instructions issued per ant, which a
+ * pair halves); one otherwise (each ant's chain of decisions is the bound).
+ * Pairs keep byte states (bucket demands all <= 254, ACO_HIP_STATE8 not 0)
+ * when asked to (ACO_HIP_STATE8=1) or when the u16 layout's workgroups
+ * need more than one round of the device's LDS: byte states halve the
+ * per-ant LDS and so raise the wavefronts resident at once.  Otherwise u16:
+ * the byte states' capped room costs one instruction per decision. */
+static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_stage,
+                        int want_pair = -1) {
+  int kmax = 1, dmax = 0;
+  for (int i : lb.insts) {
+    kmax = std::max(kmax, c->inst[i].k);
+    dmax = std::max(dmax, c->inst_dmax[i]);
+  }
+  int pair = 0;
+  if (kmax <= 32) {
+    if (want_pair < 0) want_pair = c->pair_mode;
+    if (want_pair > 0) pair = 1;
+    else if (want_pair < 0)
+      pair = (long long)lb.cols.size() * c->prm.ants > 4LL * c->num_cu ? 1 : 0;
+  }
+  const bool b8_ok = pair && dmax <= 254 && c->b8_mode != 0;
+  if (b8_ok && c->b8_mode > 0) return shape_for(c, lb, W_req, want_stage, pair, 1);
+  LaunchBucket t16 = lb;
+  const int rc = shape_for(c, t16, W_req, want_stage, pair, 0);
+  if (b8_ok) {
+    bool widen = rc != ACO_HIP_OK;
+    if (!widen) {
+      const long long groups = (long long)t16.cols.size() * t16.groups;
+      widen = groups > (long long)c->num_cu * (long long)(LDS_LIMIT / t16.lds);
+    }
+    LaunchBucket t8 = lb;
+    if (widen && shape_for(c, t8, W_req, want_stage, pair, 1) == ACO_HIP_OK) {
+      lb = t8;
+      c->err[0] = 0;
+      return ACO_HIP_OK;
+    }
+  }
+  if (rc) return rc;
+  lb = t16;
   return ACO_HIP_OK;
 }
 
@@ -343,6 +383,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   c->num_col = num * p->colonies;
   c->inst.assign(num, InstDesc{});
   c->inst_n1.assign(num, 0);
+  c->inst_dmax.assign(num, 0);
   int dmax = 0;
   size_t node_off = 0, cand_off = 0, slot_off = 0;
   int row_base = 0;
@@ -359,6 +400,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "instance %d: depot demand must be 0", i);
     int32_t xmin = a.x[0], xmax = a.x[0], ymin = a.y[0], ymax = a.y[0];
     long long total = 0;
+    int dmax_i = 0;
     for (int j = 0; j <= a.n; ++j) {
       if (a.x[j] < 0 || a.x[j] > 32767 || a.y[j] < 0 || a.y[j] > 32767)
         return set_err(c, ACO_HIP_ERR_INVALID_INPUT,
@@ -366,6 +408,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       if (a.demand[j] < 0 || a.demand[j] > 32767)
         return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "instance %d: demand out of [0, 32767]", i);
       total += j ? a.demand[j] : 0;
+      dmax_i = std::max(dmax_i, (int)a.demand[j]);
       xmin = std::min(xmin, a.x[j]); xmax = std::max(xmax, a.x[j]);
       ymin = std::min(ymin, a.y[j]); ymax = std::max(ymax, a.y[j]);
     }
@@ -394,6 +437,7 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
     d.seed = a.seed;
     d.fs_stride = fs_stride_of(a.n + 1);
     c->inst_n1[i] = a.n + 1;
+    c->inst_dmax[i] = dmax_i;
     node_off += ((size_t)a.n + 1 + 3) & ~(size_t)3;
     /* 64 padding entries: every lane of a wavefront reads row n in bounds */
     cand_off += (((size_t)a.n + 1) * d.k + 64 + 7) & ~(size_t)7;
@@ -509,6 +553,8 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
   }
   c->pair_mode = env_int("ACO_HIP_PAIR", -1);
   c->ring_mode = env_int("ACO_HIP_TOUR_RING", -1);
+  c->b8_mode = env_int("ACO_HIP_STATE8", -1);
+  c->xcd_mode = env_int("ACO_HIP_XCD", -1);
   const int W0 = p->ants_per_group > 0 ? p->ants_per_group : env_int("ACO_HIP_ANTS_PER_GROUP", 4);
   int want_stage = p->stage_lds;
   if (want_stage < 0) want_stage = env_int("ACO_HIP_STAGE_LDS", -1);
@@ -695,18 +741,20 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       (const void *)k_construct<false, true, R, false, F>,                                     \
       (const void *)k_construct<true, false, R, true, F>,                                      \
       (const void *)k_construct<false, false, R, true, F>
-#define ACO_PAIR_VARIANTS(F, RG)                                                                \
-  (const void *)k_construct_pair<true, false, false, F, RG>,                                   \
-      (const void *)k_construct_pair<false, false, false, F, RG>,                              \
-      (const void *)k_construct_pair<true, true, false, F, RG>,                                \
-      (const void *)k_construct_pair<false, true, false, F, RG>,                               \
-      (const void *)k_construct_pair<true, false, true, F, RG>,                                \
-      (const void *)k_construct_pair<false, false, true, F, RG>
+#define ACO_PAIR_VARIANTS(F, RG, B8)                                                            \
+  (const void *)k_construct_pair<true, false, false, F, RG, B8>,                               \
+      (const void *)k_construct_pair<false, false, false, F, RG, B8>,                          \
+      (const void *)k_construct_pair<true, true, false, F, RG, B8>,                            \
+      (const void *)k_construct_pair<false, true, false, F, RG, B8>,                           \
+      (const void *)k_construct_pair<true, false, true, F, RG, B8>,                            \
+      (const void *)k_construct_pair<false, false, true, F, RG, B8>
   const void *variants[] = {ACO_VARIANTS(1, true),  ACO_VARIANTS(2, true),  ACO_VARIANTS(3, true),
                             ACO_VARIANTS(4, true),  ACO_VARIANTS(1, false), ACO_VARIANTS(2, false),
                             ACO_VARIANTS(3, false), ACO_VARIANTS(4, false),
-                            ACO_PAIR_VARIANTS(true, false), ACO_PAIR_VARIANTS(false, false),
-                            ACO_PAIR_VARIANTS(true, true), ACO_PAIR_VARIANTS(false, true)};
+                            ACO_PAIR_VARIANTS(true, false, false), ACO_PAIR_VARIANTS(false, false, false),
+                            ACO_PAIR_VARIANTS(true, true, false), ACO_PAIR_VARIANTS(false, true, false),
+                            ACO_PAIR_VARIANTS(true, false, true), ACO_PAIR_VARIANTS(false, false, true),
+                            ACO_PAIR_VARIANTS(true, true, true), ACO_PAIR_VARIANTS(false, true, true)};
 #undef ACO_VARIANTS
 #undef ACO_PAIR_VARIANTS
   for (const void *f : variants)
@@ -736,6 +784,7 @@ static void bucket_info(const LaunchBucket &lb, AcoHipBucketInfo *info) {
   info->lds_bytes = (int)lb.lds;
   info->tuned_ms = lb.tuned_ms;
   info->ants_per_wave = lb.pair ? 2 : 1;
+  info->state_bytes = lb.pair && lb.b8 ? 1 : 2;
 }
 
 extern "C" int aco_hip_get_bucket(const AcoHipContext *c, int b, AcoHipBucketInfo *info) {
@@ -766,7 +815,7 @@ extern "C" int aco_hip_set_bucket_shape(AcoHipContext *c, int b, int apg, int st
 
 extern "C" int aco_hip_set_bucket_layout(AcoHipContext *c, int b, int apg, int stage, int apw) {
   if (!c || !c->loaded) return ACO_HIP_ERR_INVALID_INPUT;
-  if (b < 0 || b >= (int)c->buckets.size() || apg < 1 || apg > 8 || stage < -1 || stage > 1 ||
+  if (b < 0 || b >= (int)c->buckets.size() || apg < 1 || apg > 16 || stage < -1 || stage > 1 ||
       apw < 0 || apw > 2)
     return set_err(c, ACO_HIP_ERR_INVALID_INPUT, "bad bucket shape request");
   LaunchBucket trial = c->buckets[b];
@@ -834,21 +883,21 @@ extern "C" int aco_hip_init_tau(AcoHipContext *c) {
 
 /* One launch of the construction variant that matches the bucket's shape:
  * ANTS = 1 (k_construct, ROWS = R) or 2 (k_construct_pair, tours in
- * registers with RING), FSM = fallback weights from the all-pairs weight
- * matrix.  Launches of several colonies keep each colony's workgroups on one
- * XCD (the counter variants keep the plain order). */
-template <int ANTS, int R, bool FSM, bool RING = false>
+ * registers with RING, byte states with B8), FSM = fallback weights from the
+ * all-pairs weight matrix.  Launches of several colonies keep each colony's
+ * workgroups on one XCD (the counter variants keep the plain order). */
+template <int ANTS, int R, bool FSM, bool RING = false, bool B8 = false>
 static void launch_variant(const AcoHipContext *c, const LaunchBucket &lb, hipStream_t st,
                            bool stats) {
   DevCtx P = c->P;
   P.eta_lds = lb.eta_lds;
   const int *cols = c->d_colmap + lb.col_off;
   dim3 grid((unsigned)(lb.cols.size() * lb.groups)), block((unsigned)(64 * lb.W));
-  const bool xcd = lb.cols.size() > 1;
+  const bool xcd = c->xcd_mode < 0 ? lb.cols.size() > 1 : c->xcd_mode > 0;
 #define ACO_GO(ST, SS, X)                                                                       \
   do {                                                                                          \
     if constexpr (ANTS == 2)                                                                    \
-      hipLaunchKernelGGL((k_construct_pair<ST, SS, X, FSM, RING>), grid, block, lb.lds, st, P,  \
+      hipLaunchKernelGGL((k_construct_pair<ST, SS, X, FSM, RING, B8>), grid, block, lb.lds, st, P, \
                          lb.groups, lb.W, cols);                                                \
     else                                                                                        \
       hipLaunchKernelGGL((k_construct<ST, SS, R, X, FSM>), grid, block, lb.lds, st, P,          \
@@ -873,7 +922,11 @@ template <bool FSM>
 static void launch_bucket_f(const AcoHipContext *c, const LaunchBucket &lb, hipStream_t st,
                             bool stats) {
   if (lb.pair) {
-    if (lb.ring)
+    if (lb.ring && lb.b8)
+      launch_variant<2, 1, FSM, true, true>(c, lb, st, stats);
+    else if (lb.b8)
+      launch_variant<2, 1, FSM, false, true>(c, lb, st, stats);
+    else if (lb.ring)
       launch_variant<2, 1, FSM, true>(c, lb, st, stats);
     else
       launch_variant<2, 1, FSM, false>(c, lb, st, stats);
@@ -940,7 +993,7 @@ extern "C" int aco_hip_autotune(AcoHipContext *c, int launches, AcoHipBucketInfo
     float best_ms = -1.0f;
     for (int pair = 0; pair <= 1; ++pair)
     for (int stage = 1; stage >= 0; --stage)
-      for (int W = 1; W <= 8; W <<= 1) {
+      for (int W = 1; W <= 16; W <<= 1) {
         if (W * (pair + 1) > c->prm.ants + pair) continue;
         LaunchBucket trial = c->buckets[b];
         if (choose_shape(c, trial, W, stage, pair) != ACO_HIP_OK) continue;

@@ -213,8 +213,9 @@ __global__ void k_pp_route_costs(const int32_t *x, const int32_t *y, int n, cons
 __global__ void k_fill_tau(uint8_t *pher, size_t bytes, uint32_t word);
 template <bool STAGE, bool STATS, int ROWS, bool XCD, bool FSM>
 __global__ void k_construct(DevCtx P, int groups_per_colony, int W, const int *cols);
-template <bool STAGE, bool STATS, bool XCD, bool FSM, bool RING>
-__global__ void k_construct_pair(DevCtx P, int groups_per_colony, int W, const int *cols);
+template <bool STAGE, bool STATS, bool XCD, bool FSM, bool RING, bool B8>
+__global__ void __launch_bounds__(B8 ? 1024 : 512) k_construct_pair(DevCtx P, int groups_per_colony,
+                                                                   int W, const int *cols);
 __global__ void k_evaporate(DevCtx P, int mat_blocks, int from_ants);
 __global__ void k_evaporate_flat(DevCtx P, const long long *item_off, const long long *slot_off,
                                  int ncol, int eta_lds, const int *cmap);
