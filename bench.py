@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # (fetch x2.0, write x1.0); the commands and figures are in
 # profiles/k_construct_n1000_m256_hbm_traffic.md (1 seed) and
 # profiles/k_construct_n1000_32seeds_hbm_traffic.md (32 seeds).
-CONSTRUCT_TRAFFIC_BYTES = {1: 22648440, 32: 763776268}
+CONSTRUCT_TRAFFIC_BYTES = {1: 22631044, 32: 781296966}
 # Instructions the construct launch issues per ant-decision at this workload
 # (VALU + SALU + LDS + branch SQ counters over decisions;
 # profiles/k_construct_n1000_m256_sq_counters.md for one seed, one ant per
@@ -42,7 +42,7 @@ CONSTRUCT_TRAFFIC_BYTES = {1: 22648440, 32: 763776268}
 # two ants per wavefront, whose wavefront makes two decisions per pass): a
 # lone wavefront issues at most one instruction per 4-cycle slot of its SIMD,
 # so 4 x this is the issue bound of one decision in shader cycles.
-CONSTRUCT_INSTS_PER_DECISION = {1: 101.4, 32: 55.9}
+CONSTRUCT_INSTS_PER_DECISION = {1: 96.5, 32: 50.2}
 
 
 def tour_bytes(n, K, k):
