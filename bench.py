@@ -226,6 +226,7 @@ def batched_leg(epochs: int, fallback_matrix: int = -1):
             "buckets": [{"n1": [b["n1_min"], b["n1_max"]], "instances": b["instances"],
                          "ants_per_group": b["ants_per_group"], "stage_lds": b["stage_lds"],
                          "ants_per_wave": b["ants_per_wave"], "state_bytes": b["state_bytes"],
+                         "lanes": b["lanes"],
                          "construct_ms": round(b["tuned_ms"], 3)} for b in shapes],
             "prepare_s": round(t_prep, 3), "autotune_s": round(t_tune, 3)}
 

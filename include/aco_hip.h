@@ -174,8 +174,10 @@ int aco_hip_get_shape(const AcoHipContext *ctx, int *cand_k, int *ants, int *tou
 /* ---- size buckets and autotuning (no reference counterpart: the reference
  *      solves one instance per launch) ---------------------------------------
  * Instances of a context are grouped by node count (n+1 up to 256, 512,
- * 1024, ...; ACO_HIP_BUCKETS=0 keeps one group); each bucket has its own
- * construct launch shape, and the buckets' launches run concurrently. */
+ * 1024, ...; ACO_HIP_BUCKETS=0 keeps one group); a group whose launch holds
+ * more than 8 ants per CU is split into two lanes (ACO_HIP_LANES forces the
+ * count).  Each bucket (a group or a lane) has its own construct launch
+ * shape and its own chain of launches, and the chains run concurrently. */
 typedef struct {
   int n1_min, n1_max;    /* node counts present in the bucket */
   int instances, colonies;
@@ -186,6 +188,7 @@ typedef struct {
   int ants_per_wave;     /* 1, or 2: two ants per wavefront (candidate lists <= 32) */
   int state_bytes;       /* bytes per node of an ant's LDS state: 2, or 1 for pairs whose
                           * instances' demands are all <= 254 */
+  int lanes;             /* buckets of its size class running as concurrent chains */
 } AcoHipBucketInfo;
 int aco_hip_num_buckets(const AcoHipContext *ctx);
 int aco_hip_get_bucket(const AcoHipContext *ctx, int bucket, AcoHipBucketInfo *info);

@@ -137,6 +137,30 @@ def test_bucket_chains_match_context_wide_updates(gpu, monkeypatch):
     b.close()
 
 
+def test_lanes_are_pure_scheduling(gpu, monkeypatch):
+    """A size class split into two lanes (two chains of launches on two
+    streams, the second starting after the first's first construct, forked
+    and joined once per graph chunk) gives exactly the tours, pheromone and
+    colony states of the same class run as one bucket."""
+    insts = [Instance.uniform(150, 3, seed=400 + i) for i in range(12)]
+    seeds = [500 + i for i in range(12)]
+    p = Params(cand_k=12, ants=256, rho=0.1)
+    out = []
+    for lanes in (2, 1):
+        monkeypatch.setenv("ACO_HIP_LANES", str(lanes))
+        with HipSolver(insts, p, seeds=seeds) as s:
+            bk = s.buckets()
+            assert len(bk) == lanes and all(b["lanes"] == lanes for b in bk)
+            s.prepare()
+            s.run(fixed_epochs=6, chunk_epochs=3, use_graph=True)
+            out.append([(s.tours(c), s.tau(c), s.colony_state(c)) for c in range(len(insts))])
+    for c, (a, b) in enumerate(zip(*out)):
+        for x, y in zip(a[0], b[0]):
+            assert np.array_equal(x, y), c
+        assert np.array_equal(a[1][0], b[1][0]) and np.array_equal(a[1][1], b[1][1]), c
+        assert a[2] == b[2], c
+
+
 def test_counter_variants_match(gpu):
     """The counter-collecting construct variants (plain workgroup order, no
     XCD renumbering) build the same tours as the production variants of a

@@ -137,6 +137,12 @@ def test_c2_thirty_two_seeds_in_one_context(gpu):
     insts = [W.c2_instance(sd) for sd in seeds]
     p = W.c2_params()
     with HipSolver(insts, p, seeds=seeds) as s:
+        # the layout the benchmark runs under the automatic policy: two lanes
+        # of 16 colonies, two ants per wavefront, rows from global memory
+        if all(os.environ.get(v, "-1") == "-1" for v in ("ACO_HIP_PAIR", "ACO_HIP_LANES")):
+            bk = s.buckets()
+            assert len(bk) == 2 and all(b["lanes"] == 2 and b["colonies"] == 16 for b in bk)
+            assert all(b["ants_per_wave"] == 2 and b["stage_lds"] == 0 for b in bk)
         s.prepare()
         s.run(fixed_epochs=2, chunk_epochs=2, use_graph=True)
         for c in range(len(seeds)):
@@ -173,7 +179,7 @@ def test_c4_batched_matches_oracle(gpu):
     with HipSolver(insts, p, seeds=seeds, cand_k=ks) as s:
         s.prepare()
         shapes = tuner.tune_shapes(s, cache=None, launches=1)
-        assert len(shapes) == 4
+        assert sorted({tuner.size_class(b["n1_max"]) for b in shapes}) == [256, 512, 1024, 2048]
         s.run(fixed_epochs=2, chunk_epochs=2, use_graph=True)
         for c, inst in enumerate(insts):
             tours, lens, cost, unvis = s.tours(c)

@@ -44,7 +44,7 @@ class AcoHipBucketInfo(C.Structure):
     _fields_ = [("n1_min", C.c_int), ("n1_max", C.c_int), ("instances", C.c_int),
                 ("colonies", C.c_int), ("ants_per_group", C.c_int), ("stage_lds", C.c_int),
                 ("lds_bytes", C.c_int), ("tuned_ms", C.c_float), ("ants_per_wave", C.c_int),
-                ("state_bytes", C.c_int)]
+                ("state_bytes", C.c_int), ("lanes", C.c_int)]
 
 
 class AcoHipParams(C.Structure):

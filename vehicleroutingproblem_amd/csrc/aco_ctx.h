@@ -31,6 +31,7 @@ struct LaunchBucket {
   int pair = 0;              /* two ants per wavefront (k_construct_pair) */
   int ring = 0;              /* pairs: tours in registers, stored in 128-byte chunks */
   int b8 = 0;                /* pairs: byte states (every demand of the bucket <= 254) */
+  int lanes = 1, lane = 0;   /* lanes of its size class (concurrent chains), and its index */
   size_t lds = 0;
   float tuned_ms = 0.0f;     /* construct time measured by the autotuner, 0 = not tuned */
   size_t off_at = 0;         /* its item offsets at d_boff[off_at], slot offsets after them */

@@ -255,7 +255,7 @@ static int shape_for(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_sta
     ring = 0;
     need = need_with(0);
     if (pair) {
-      const long long groups = (long long)lb.cols.size() * ((c->prm.ants + 2 * W - 1) / (2 * W));
+      const long long groups = (long long)lb.cols.size() * lb.lanes * ((c->prm.ants + 2 * W - 1) / (2 * W));
       const long long per_cu = std::max(1LL, std::min((groups + c->num_cu - 1) / c->num_cu,
                                                       (long long)(32 / W)));
       if (c->ring_mode > 0 || (c->ring_mode < 0 && (long long)need * per_cu > (long long)LDS_LIMIT)) {
@@ -276,7 +276,7 @@ static int shape_for(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_sta
    * latency (n=1000, 32 colonies of 256 ants: 1.09 ms per epoch against
    * 2.47 ms staged) */
   if (want_stage < 0 && need <= LDS_LIMIT) {
-    const long long groups = (long long)lb.cols.size() * ((c->prm.ants + W * apw - 1) / (W * apw));
+    const long long groups = (long long)lb.cols.size() * lb.lanes * ((c->prm.ants + W * apw - 1) / (W * apw));
     const long long per_round = (long long)c->num_cu * (long long)(LDS_LIMIT / need);
     if (groups > per_round) need = LDS_LIMIT + 1; /* take the global-row branch below */
   }
@@ -331,7 +331,7 @@ static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_
     if (want_pair < 0) want_pair = c->pair_mode;
     if (want_pair > 0) pair = 1;
     else if (want_pair < 0)
-      pair = (long long)lb.cols.size() * c->prm.ants > 4LL * c->num_cu ? 1 : 0;
+      pair = (long long)lb.cols.size() * lb.lanes * c->prm.ants > 4LL * c->num_cu ? 1 : 0;
   }
   const bool b8_ok = pair && dmax <= 254 && c->b8_mode != 0;
   if (b8_ok && c->b8_mode > 0) return shape_for(c, lb, W_req, want_stage, pair, 1);
@@ -340,7 +340,7 @@ static int choose_shape(AcoHipContext *c, LaunchBucket &lb, int W_req, int want_
   if (b8_ok) {
     bool widen = rc != ACO_HIP_OK;
     if (!widen) {
-      const long long groups = (long long)t16.cols.size() * t16.groups;
+      const long long groups = (long long)t16.cols.size() * t16.lanes * t16.groups;
       widen = groups > (long long)c->num_cu * (long long)(LDS_LIMIT / t16.lds);
     }
     LaunchBucket t8 = lb;
@@ -541,6 +541,36 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       lb.insts.push_back(i);
       lb.n1_lo = std::min(lb.n1_lo, c->inst_n1[i]);
       for (int cc = 0; cc < p->colonies; ++cc) lb.cols.push_back(i * p->colonies + cc);
+    }
+    /* lanes: a size class whose launch holds more than 8 ants per CU and at
+     * least two instances is split into two lanes (ACO_HIP_LANES forces the
+     * count), each a bucket with its own chain of construct, evaporation and
+     * deposit on its own stream.  A lane's update kernels, which leave most
+     * of the device idle, then overlap the other lane's construction. */
+    const int lanes_env = env_int("ACO_HIP_LANES", -1);
+    {
+      std::vector<LaunchBucket> split_b;
+      for (const LaunchBucket &lb : c->buckets) {
+        const int ni = (int)lb.insts.size();
+        const long long ants = (long long)lb.cols.size() * p->ants;
+        int L = lanes_env > 0 ? lanes_env : (ants > 8LL * c->num_cu ? 2 : 1);
+        L = std::max(1, std::min(L, ni));
+        for (int l = 0; l < L; ++l) {
+          LaunchBucket nb;
+          nb.n1_hi = lb.n1_hi;
+          nb.n1_lo = 1 << 30;
+          nb.lanes = L;
+          nb.lane = l;
+          for (int q = l * ni / L; q < (l + 1) * ni / L; ++q) {
+            const int i = lb.insts[q];
+            nb.insts.push_back(i);
+            nb.n1_lo = std::min(nb.n1_lo, c->inst_n1[i]);
+            for (int cc = 0; cc < p->colonies; ++cc) nb.cols.push_back(i * p->colonies + cc);
+          }
+          split_b.push_back(nb);
+        }
+      }
+      c->buckets.swap(split_b);
     }
     int off = 0;
     for (LaunchBucket &lb : c->buckets) {
@@ -785,6 +815,7 @@ static void bucket_info(const LaunchBucket &lb, AcoHipBucketInfo *info) {
   info->tuned_ms = lb.tuned_ms;
   info->ants_per_wave = lb.pair ? 2 : 1;
   info->state_bytes = lb.pair && lb.b8 ? 1 : 2;
+  info->lanes = lb.lanes;
 }
 
 extern "C" int aco_hip_get_bucket(const AcoHipContext *c, int b, AcoHipBucketInfo *info) {
@@ -999,6 +1030,12 @@ extern "C" int aco_hip_autotune(AcoHipContext *c, int launches, AcoHipBucketInfo
         if (choose_shape(c, trial, W, stage, pair) != ACO_HIP_OK) continue;
         /* does not fit as asked (or no pairs for lists above 32) */
         if (trial.W != W || trial.stage != stage || trial.pair != pair) continue;
+        /* a lane is timed alone, but runs beside its class's other lanes:
+         * staged rows only where all of them fit one LDS round */
+        if (trial.stage && trial.lanes > 1 &&
+            (long long)trial.cols.size() * trial.lanes * trial.groups >
+                (long long)c->num_cu * (long long)(LDS_LIMIT / trial.lds))
+          continue;
         launch_bucket(c, trial, c->stream, stats);          /* warm caches */
         HIPC(c, hipEventRecord(a, c->stream));
         for (int r = 0; r < launches; ++r) launch_bucket(c, trial, c->stream, stats);
@@ -1108,28 +1145,36 @@ static int launch_select_deposit(AcoHipContext *c) {
   return ACO_HIP_OK;
 }
 
-/* One epoch of a multi-bucket context as independent chains: every size
- * bucket constructs, evaporates and deposits its own colonies on its own
- * stream (bucket 0 on the context stream), forked from and joined back to
- * the context stream.  A bucket's update kernels then run while larger
- * buckets are still constructing.  Colonies never share state, so the result
- * is the same as with the context-wide update kernels. */
-static int launch_epoch_pipelined(AcoHipContext *c) {
+/* `count` epochs of a multi-bucket context as independent chains: every
+ * bucket (a size class, or a lane of one) constructs, evaporates and
+ * deposits its own colonies on its own stream (bucket 0 on the context
+ * stream), forked from the context stream once and joined back after the
+ * last epoch, so a chain runs ahead of the others as far as it is faster.  A
+ * bucket's update kernels then run while other buckets construct.  Colonies
+ * never share state, so the result is the same as with the context-wide
+ * update kernels. */
+static int launch_epoch_pipelined(AcoHipContext *c, int count) {
   const bool stats = c->P.stats != nullptr;
   const size_t nb = c->buckets.size();
   HIPC(c, hipEventRecord(c->bevents[0], c->stream));
   for (size_t b = 1; b < nb; ++b) HIPC(c, hipStreamWaitEvent(c->bstreams[b - 1], c->bevents[0], 0));
-  for (size_t b = 0; b < nb; ++b) {
-    const LaunchBucket &lb = c->buckets[b];
-    hipStream_t st = b == 0 ? c->stream : c->bstreams[b - 1];
-    const int nc = (int)lb.cols.size();
-    const int *cmap = c->d_colmap + lb.col_off;
-    launch_bucket(c, lb, st, stats);
-    enqueue_evap_flat(c, st, c->d_boff + lb.off_at, c->d_boff + lb.off_at + nc + 1, nc, cmap,
-                      lb.flat_total);
-    hipLaunchKernelGGL(k_select_deposit, dim3(lb.sd_blocks, nc), dim3(256), c->sd_lds, st, c->P,
-                       c->sd_eta, cmap);
-  }
+  for (int e = 0; e < count; ++e)
+    for (size_t b = 0; b < nb; ++b) {
+      const LaunchBucket &lb = c->buckets[b];
+      hipStream_t st = b == 0 ? c->stream : c->bstreams[b - 1];
+      const int nc = (int)lb.cols.size();
+      const int *cmap = c->d_colmap + lb.col_off;
+      /* the other lanes of a class start after lane 0's first construct, so
+       * that the lanes' update phases fall in each other's constructions */
+      const size_t b0 = b - (size_t)lb.lane;
+      if (e == 0 && lb.lane > 0) HIPC(c, hipStreamWaitEvent(st, c->bevents[b0], 0));
+      launch_bucket(c, lb, st, stats);
+      if (e == 0 && lb.lane == 0 && lb.lanes > 1) HIPC(c, hipEventRecord(c->bevents[b0], st));
+      enqueue_evap_flat(c, st, c->d_boff + lb.off_at, c->d_boff + lb.off_at + nc + 1, nc, cmap,
+                        lb.flat_total);
+      hipLaunchKernelGGL(k_select_deposit, dim3(lb.sd_blocks, nc), dim3(256), c->sd_lds, st,
+                         c->P, c->sd_eta, cmap);
+    }
   for (size_t b = 1; b < nb; ++b) {
     HIPC(c, hipEventRecord(c->bevents[b], c->bstreams[b - 1]));
     HIPC(c, hipStreamWaitEvent(c->stream, c->bevents[b], 0));
@@ -1178,8 +1223,8 @@ static int launch_epochs(AcoHipContext *c, int count) {
      * tables of that kernel use the fused evaporate + single-workgroup
      * update kernel instead */
     if (c->sd_lds && c->buckets.size() > 1 && !c->timing && !c->serial_updates) {
-      if ((rc = launch_epoch_pipelined(c))) return rc;
-      continue;
+      if ((rc = launch_epoch_pipelined(c, count))) return rc;
+      break;
     }
     if ((rc = launch_construct(c))) return rc;
     if (c->sd_lds) {
