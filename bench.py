@@ -476,6 +476,7 @@ def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int =
     epochs = [st["epoch"] - warmup for st in states]
     feasible_last = sum(float((s.tours(i)[3] == 0).mean()) for i in range(len(seeds))) / len(seeds)
     shape = (s.ants_per_group, s.stage_lds, s.buckets()[0]["ants_per_wave"])
+    lanes = len(s.buckets())
     s.close()
 
     def replay(timing: bool):
@@ -515,13 +516,19 @@ def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int =
         "feasible_ant_fraction_last_epoch": round(feasible_last, 4),
         "best_cost": [st["global_best_cost"] for st in states][:8],
         "ants_per_group": shape[0], "stage_lds": shape[1], "ants_per_wave": shape[2],
+        "lanes": lanes,
         "roofline": {"bound": "hbm", "kernel": "k_construct_pair" if shape[2] == 2 else "k_construct",
                      "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": CONSTRUCT_TRAFFIC_BYTES.get(len(seeds)),
                      "avg_launch_ms": round(construct_ms, 5),
-                     "bytes_per_launch": bytes_per_launch},
+                     "bytes_per_launch": bytes_per_launch,
+                     "concurrent_launches": lanes,
+                     "note": ("the epoch's construct phase: one launch per lane, the lanes' launches "
+                              "running concurrently on their own streams; avg_launch_ms spans "
+                              "all of them (first start to last end) and bytes_per_launch and "
+                              "traffic are their sum")},
         "kernel_ms_per_epoch": {
             "construct": round(construct_ms, 5),
             "evaporate": round(t["evaporate_ms"] / launches, 5),
@@ -659,7 +666,8 @@ def main():
                    "rho": W.C2_RHO, "seeds": len(seeds),
                    "ants_per_group": head.pop("ants_per_group"),
                    "stage_lds": head.pop("stage_lds"),
-                   "ants_per_wave": head.pop("ants_per_wave")},
+                   "ants_per_wave": head.pop("ants_per_wave"),
+                   "lanes": head.pop("lanes")},
         "roofline": head.pop("roofline"),
     }
     out.update(head)
