@@ -28,13 +28,13 @@ from vehicleroutingproblem_amd import workloads as W  # noqa: E402
 
 N_CUST, K_VEH, ANTS, CAND_K, RHO = 1000, 8, 256, 20, 0.1
 HBM_PEAK_GBS = 8000.0
-# HBM bytes one k_construct launch of the n=1000 workload moves, by number of
-# seeds in the launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
-# this bench, each scaled by what the counter reads for a 256 MiB device copy
-# (fetch x2.0, write x1.0); the commands and figures are in
-# profiles/k_construct_n1000_m256_hbm_traffic.md (1 seed) and
-# profiles/k_construct_n1000_32seeds_hbm_traffic.md (32 seeds).
-CONSTRUCT_TRAFFIC_BYTES = {1: 22631044, 32: 781296966}
+# HBM bytes the construct phase of one epoch of the n=1000 workload moves, by
+# number of seeds (1: one launch; 32: one launch per lane, two lanes), from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes over this bench, each scaled by
+# what the counter reads for a 256 MiB device copy (fetch x2.0, write x1.0);
+# the commands and figures are in profiles/k_construct_n1000_m256_hbm_traffic.md
+# (1 seed) and profiles/k_construct_n1000_32seeds_hbm_traffic.md (32 seeds).
+CONSTRUCT_TRAFFIC_BYTES = {1: 22631044, 32: 505230354}
 # Instructions the construct launch issues per ant-decision at this workload
 # (VALU + SALU + LDS + branch SQ counters over decisions;
 # profiles/k_construct_n1000_m256_sq_counters.md for one seed, one ant per

@@ -174,10 +174,11 @@ int aco_hip_get_shape(const AcoHipContext *ctx, int *cand_k, int *ants, int *tou
 /* ---- size buckets and autotuning (no reference counterpart: the reference
  *      solves one instance per launch) ---------------------------------------
  * Instances of a context are grouped by node count (n+1 up to 256, 512,
- * 1024, ...; ACO_HIP_BUCKETS=0 keeps one group); a group whose launch holds
- * more than 8 ants per CU is split into two lanes (ACO_HIP_LANES forces the
- * count).  Each bucket (a group or a lane) has its own construct launch
- * shape and its own chain of launches, and the chains run concurrently. */
+ * 1024, ...; ACO_HIP_BUCKETS=0 keeps one group); a context with a single
+ * group whose launch holds more than 8 ants per CU splits it into two lanes
+ * (ACO_HIP_LANES forces the count for every group).  Each bucket (a group or
+ * a lane) has its own construct launch shape and its own chain of launches,
+ * and the chains run concurrently. */
 typedef struct {
   int n1_min, n1_max;    /* node counts present in the bucket */
   int instances, colonies;

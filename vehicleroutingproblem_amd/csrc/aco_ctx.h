@@ -19,6 +19,9 @@
 struct TimingRec {
   int kind;
   hipEvent_t a, b;
+  /* extra end events of a phase of concurrent launches (b is the first):
+   * the phase lasts from a to the latest of them */
+  std::vector<hipEvent_t> more;
 };
 
 /* One size class of instances and its construct launch shape. */

@@ -135,6 +135,12 @@ static int resolve_timing(AcoHipContext *c) {
   for (TimingRec &r : c->pending) {
     float ms = 0.0f;
     HIPC(c, hipEventElapsedTime(&ms, r.a, r.b));
+    for (hipEvent_t e : r.more) {
+      float m2 = 0.0f;
+      HIPC(c, hipEventElapsedTime(&m2, r.a, e));
+      ms = std::max(ms, m2);
+      c->free_events.push_back(e);
+    }
     switch (r.kind) {
       case T_CONSTRUCT: c->t.construct_ms += ms; c->t.construct_launches++; break;
       case T_BEST: c->t.best_ms += ms; c->t.best_launches++; break;
@@ -163,7 +169,7 @@ struct Timed {
   ~Timed() {
     if (c->timing && a && b) {
       (void)hipEventRecord(b, c->stream);
-      c->pending.push_back({kind, a, b});
+      c->pending.push_back({kind, a, b, {}});
       if (c->pending.size() >= 4096) (void)resolve_timing(c);
     }
   }
@@ -202,6 +208,7 @@ extern "C" void aco_hip_destroy(AcoHipContext *c) {
   for (TimingRec &r : c->pending) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
+    for (hipEvent_t e : r.more) (void)hipEventDestroy(e);
   }
   for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
   free_pools(c);
@@ -542,18 +549,22 @@ extern "C" int aco_hip_load(AcoHipContext *c, const AcoHipInstance *in, int num,
       lb.n1_lo = std::min(lb.n1_lo, c->inst_n1[i]);
       for (int cc = 0; cc < p->colonies; ++cc) lb.cols.push_back(i * p->colonies + cc);
     }
-    /* lanes: a size class whose launch holds more than 8 ants per CU and at
-     * least two instances is split into two lanes (ACO_HIP_LANES forces the
-     * count), each a bucket with its own chain of construct, evaporation and
-     * deposit on its own stream.  A lane's update kernels, which leave most
-     * of the device idle, then overlap the other lane's construction. */
+    /* lanes: the size class of a context with one class, when its launch
+     * holds more than 8 ants per CU and at least two instances, is split into
+     * two lanes (ACO_HIP_LANES forces the count for every class), each a
+     * bucket with its own chain of construct, evaporation and deposit on its
+     * own stream.  A lane's update kernels, which leave most of the device
+     * idle, then overlap the other lane's construction.  Several classes
+     * already overlap their chains (and more streams than the device's
+     * hardware queues gain nothing), so they are not split. */
     const int lanes_env = env_int("ACO_HIP_LANES", -1);
     {
       std::vector<LaunchBucket> split_b;
+      const bool one_class = c->buckets.size() == 1;
       for (const LaunchBucket &lb : c->buckets) {
         const int ni = (int)lb.insts.size();
         const long long ants = (long long)lb.cols.size() * p->ants;
-        int L = lanes_env > 0 ? lanes_env : (ants > 8LL * c->num_cu ? 2 : 1);
+        int L = lanes_env > 0 ? lanes_env : (one_class && ants > 8LL * c->num_cu ? 2 : 1);
         L = std::max(1, std::min(L, ni));
         for (int l = 0; l < L; ++l) {
           LaunchBucket nb;
@@ -982,21 +993,48 @@ static void launch_bucket(AcoHipContext *c, const LaunchBucket &lb, hipStream_t 
  * others on their own streams forked from it and joined back, so the
  * buckets run side by side (also inside a captured graph). */
 static int launch_construct(AcoHipContext *c) {
-  {
+  const bool stats = c->P.stats != nullptr;
+  const size_t nb = c->buckets.size();
+  if (nb == 1) {
     Timed tm(c, T_CONSTRUCT);
-    const bool stats = c->P.stats != nullptr;
-    const size_t nb = c->buckets.size();
-    if (nb > 1) {
-      HIPC(c, hipEventRecord(c->bevents[0], c->stream));
-      for (size_t b = 1; b < nb; ++b) HIPC(c, hipStreamWaitEvent(c->bstreams[b - 1], c->bevents[0], 0));
+    launch_bucket(c, c->buckets[0], c->stream, stats);
+  } else {
+    /* the buckets' launches run concurrently on their own streams; timed,
+     * the phase lasts from bucket 0's start to the latest end, each end
+     * recorded on its own stream (events around the fork and the join
+     * would add their cross-stream waits to it) */
+    HIPC(c, hipEventRecord(c->bevents[0], c->stream));
+    for (size_t b = 1; b < nb; ++b) HIPC(c, hipStreamWaitEvent(c->bstreams[b - 1], c->bevents[0], 0));
+    const bool timed = c->timing;
+    TimingRec rec{T_CONSTRUCT, nullptr, nullptr, {}};
+    if (timed) {
+      rec.a = get_event(c);
+      rec.b = get_event(c);
+      if (!rec.a || !rec.b) return set_err(c, ACO_HIP_ERR_BACKEND, "hipEventCreate failed");
+      HIPC(c, hipEventRecord(rec.a, c->stream));
     }
     launch_bucket(c, c->buckets[0], c->stream, stats);
-    for (size_t b = 1; b < nb; ++b) launch_bucket(c, c->buckets[b], c->bstreams[b - 1], stats);
-    if (nb > 1)
-      for (size_t b = 1; b < nb; ++b) {
-        HIPC(c, hipEventRecord(c->bevents[b], c->bstreams[b - 1]));
-        HIPC(c, hipStreamWaitEvent(c->stream, c->bevents[b], 0));
+    if (timed) HIPC(c, hipEventRecord(rec.b, c->stream));
+    for (size_t b = 1; b < nb; ++b) {
+      launch_bucket(c, c->buckets[b], c->bstreams[b - 1], stats);
+      if (timed) {
+        hipEvent_t e = get_event(c);
+        if (!e) return set_err(c, ACO_HIP_ERR_BACKEND, "hipEventCreate failed");
+        HIPC(c, hipEventRecord(e, c->bstreams[b - 1]));
+        rec.more.push_back(e);
       }
+    }
+    for (size_t b = 1; b < nb; ++b) {
+      HIPC(c, hipEventRecord(c->bevents[b], c->bstreams[b - 1]));
+      HIPC(c, hipStreamWaitEvent(c->stream, c->bevents[b], 0));
+    }
+    if (timed) {
+      c->pending.push_back(std::move(rec));
+      if (c->pending.size() >= 4096) {
+        const int rc = resolve_timing(c);
+        if (rc) return rc;
+      }
+    }
   }
   LAUNCHED(c, "k_construct");
   return ACO_HIP_OK;
