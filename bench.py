@@ -528,7 +528,8 @@ def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int =
                      "note": ("the epoch's construct phase: one launch per lane, the lanes' launches "
                               "running concurrently on their own streams; avg_launch_ms spans "
                               "all of them (first start to last end) and bytes_per_launch and "
-                              "traffic are their sum")},
+                              "traffic are their sum") if lanes > 1 else
+                             "one construct launch per epoch"},
         "kernel_ms_per_epoch": {
             "construct": round(construct_ms, 5),
             "evaporate": round(t["evaporate_ms"] / launches, 5),
