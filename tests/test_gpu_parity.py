@@ -195,6 +195,36 @@ def test_pair_layout_matches_oracle(gpu, name, stage, fm, dist, ring, monkeypatc
     s.close()
 
 
+@pytest.mark.parametrize("n,K,seed", [(24, 2, 2), (40, 3, 3)])
+def test_epochs_without_a_feasible_ant(gpu, n, K, seed):
+    """Heavy demands (U[30..70] or U[20..60]) and a capacity with no slack
+    leave most epochs without a feasible ant: the reference then neither
+    evaporates nor deposits (aco_cuda.cu:355-396).  Tours, colony state and
+    pheromone equal the oracle's after every epoch, and both kinds of epoch
+    occur."""
+    dmin = 30 if n == 24 else 20
+    dmax = 70 if n == 24 else 60
+    inst = Instance.uniform(n, K, dmin=dmin, dmax=dmax, slack_pct=100, seed=seed)
+    p = Params(cand_k=8, ants=4, rho=0.3)
+    o = O.Oracle(inst, cand_k=8, ants=4, rho=0.3, seed=7)
+    fe = []
+    with HipSolver(inst, p, seeds=[7]) as s:
+        s.prepare()
+        for e in range(10):
+            s.epochs(1)
+            o.epochs(1)
+            _compare_tours(s, o)
+            gs, os_ = s.colony_state(), o.state()
+            for key in ("epoch", "best_ant", "feasible_ants", "global_best_cost", "iter_since_best"):
+                assert gs[key] == os_[key], (e, key, gs[key], os_[key])
+            tg, cg = s.tau()
+            to, co = o.tau()
+            assert np.array_equal(tg, to) and np.array_equal(cg, co), e
+            fe.append(gs["feasible_ants"])
+    o.close()
+    assert 0 in fe and max(fe) > 0, fe  # both kinds of epoch occurred
+
+
 @pytest.mark.parametrize("ring", [0, 1])
 @pytest.mark.parametrize("ants,apg", [(33, 4), (7, 1), (2, 3), (1, 2)])
 def test_pair_layout_ragged_ant_counts(gpu, ants, apg, ring, monkeypatch):
