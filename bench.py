@@ -455,7 +455,8 @@ def c2_run(seeds, steps: int, warmup: int, chunk_cap: int, ants_per_group: int =
     seeds = list(seeds)
     insts = [W.c2_instance(x) for x in seeds]
     p = W.c2_params(ants_per_group=ants_per_group)
-    chunk = div_chunk(steps, chunk_cap)
+    # the library replays at most 64 epochs per graph (its per-epoch progress ring)
+    chunk = div_chunk(steps, min(chunk_cap, 64))
 
     def fresh():
         r_ = HipSolver(insts, p, seeds=seeds)
@@ -560,7 +561,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ants-per-group", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=64,
-                    help="max epochs per graph replay (the largest divisor of --steps up to it)")
+                    help="max epochs per graph replay (the largest divisor of --steps up to it, "
+                         "at most 64)")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-seed n=1000 leg")
     ap.add_argument("--no-batch", action="store_true",
